@@ -1,0 +1,234 @@
+/*
+ * htr_oracle.h -- CPU oracle for the HTR per-cell right-hand-side hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (stanfordhpccenter/HTR-solver, files cited per function in the .c
+ * sources).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker / the timed CPU baseline.  The
+ * product library (htr-solver_amd/csrc) never links or calls it.
+ *
+ * Data model (oracle side):
+ *   - every field is a dense structure-of-arrays block over the whole
+ *     "Fluid_bounds" box of nx*ny*nz cells (ghost cells included):
+ *         value(f, comp, i, j, k) = f[comp*N + i + nx*(j + ny*k)],  N = nx*ny*nz
+ *   - the execution rectangle [lo, hi] (inclusive) is the index space the
+ *     reference task would run over (src/prometeo_rhs.cc:70-71).
+ *   - periodic index wrapping follows warpPeriodic over the full box
+ *     (src/Utils/PointDomain_helper.hpp:157-188).
+ *
+ * Parity status of each function is recorded next to its declaration and in
+ * the header of the source file that implements it.
+ */
+#ifndef HTR_ORACLE_H
+#define HTR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O_NSP_MAX   40
+#define O_NEQ_MAX   (O_NSP_MAX + 4)
+#define O_NREAC_MAX 256
+#define O_MAXR      4
+#define O_MAXP      4
+#define O_MAXTB     40
+
+/* reaction kinds, evaluated in this order inside GetProductionRates
+ * (src/Mixtures/MultiComponent.inl:371-417) */
+#define O_REAC_STD      0
+#define O_REAC_THIRDB   1
+#define O_REAC_FALLOFF  2
+
+#define O_F_LINDEMANN 0
+#define O_F_TROE2     1
+#define O_F_TROE3     2
+#define O_F_SRI       3
+
+#define O_GEOM_ATOM      0
+#define O_GEOM_LINEAR    1
+#define O_GEOM_NONLINEAR 3
+
+#define O_VISC_CONSTANT   0
+#define O_VISC_POWERLAW   1
+#define O_VISC_SUTHERLAND 2
+
+typedef struct {
+   float  TSwitch1, TSwitch2, TMin, TMax;   /* float, as cpCoefficients (Species.hpp) */
+   int    nPoly;                            /* 2 or 3 NASA ranges */
+   double cpH[9], cpM[9], cpL[9];
+} o_cpcoeff;
+
+typedef struct {
+   int    geom;
+   double sigma, kbOveps, mu, alpha, Z298;
+} o_diffcoeff;
+
+typedef struct {
+   double      W;
+   o_cpcoeff   cp;
+   o_diffcoeff dc;
+} o_spec;
+
+typedef struct { int ind; float nu;  } o_stoich;
+typedef struct { int ind; float eff; } o_thirdb;
+typedef struct { double A; float n; float EovR; } o_arrhenius;
+
+typedef struct {
+   int kind;                  /* O_REAC_* */
+   int has_backward;
+   int Ne, Np, Ntb;
+   o_arrhenius arr;           /* standard / third-body: the rate; fall-off: high-pressure */
+   o_arrhenius arrL;          /* fall-off low-pressure */
+   o_stoich ed[O_MAXR];
+   o_stoich pd[O_MAXP];
+   o_thirdb tb[O_MAXTB];
+   int    Ftype;
+   float  alpha;              /* Troe alpha */
+   double T1, T2, T3;         /* Troe temperatures */
+   float  sA, sB, sC, sD, sE; /* SRI */
+} o_reac;
+
+typedef struct {
+   int constprop;             /* 1: ConstPropMix (src/Mixtures/ConstPropMix.inl) */
+   int nSpec;
+   int nReac;
+   /* ConstPropMix parameters */
+   double R, gamma, Prandtl;
+   int    viscModel;
+   double constVisc, plTRef, plVRef, suSRef, suTRef, suVRef;
+   /* multicomponent data */
+   o_spec sp[O_NSP_MAX];
+   o_reac rc[O_NREAC_MAX];
+   /* reference quantities (src/Mixtures/MultiComponent.hpp:124-144) */
+   double PRef, TRef, XiRef[O_NSP_MAX];
+   double MixWRef, iMixWRef, rhoRef, ieRef, iuRef, iCpRef, imuRef, ilamRef, iDiRef, iwiRef, iKiRef, Eps0;
+   double TMax, TMin;
+} o_mix;
+
+typedef struct {
+   int nx, ny, nz;            /* Fluid_bounds extents (lo = 0) */
+   int lo[3], hi[3];          /* execution rectangle, inclusive */
+} o_grid;
+
+typedef struct {
+   const int     *nType[3];
+   const double  *m_e[3], *m_d[3], *m_s[3];
+   double *rho, *mu, *lam, *Di, *SoS;
+   double *pressure, *temperature, *MassFracs, *MolarFracs, *velocity;
+   double *Conserved, *Conserved_old, *Conserved_t, *Conserved_t_old;
+   uint8_t *shockSensor[3];
+   double *DucrosSensor;
+} o_fields;
+
+/* ---- mixture builders --------------------------------------------------- */
+int    o_mix_sizeof(void);
+void   o_mix_init_constprop(o_mix *m, double R, double gamma, double Prandtl,
+                            int viscModel, double p0, double p1, double p2);
+/* AirMix (src/Mixtures/AirMix.hpp); XiRef given for species names by index
+ * into {N2, O2, NO, N, O}; nref entries */
+void   o_mix_init_airmix(o_mix *m, double PRef, double TRef, double LRef,
+                         int nref, const int *ref_idx, const double *ref_X);
+/* generic multicomponent mixture from flat arrays:
+ * W[ns], cp floats [ns*4] (TSwitch1,TSwitch2,TMin,TMax), nPoly[ns],
+ * cpH/cpM/cpL [ns*9], geom[ns], dc[ns*5] (sigma,kbOveps,mu,alpha,Z298) */
+void   o_mix_init_species(o_mix *m, int ns, const double *W, const float *cpT,
+                          const int *nPoly, const double *cpH, const double *cpM,
+                          const double *cpL, const int *geom, const double *dc);
+/* append a standard or third-body reaction (fall-off via o_mix_add_falloff) */
+void   o_mix_add_reaction(o_mix *m, int kind, double A, float n, float EovR,
+                          int has_backward, int Ne, const int *ed_ind, const float *ed_nu,
+                          int Np, const int *pd_ind, const float *pd_nu,
+                          int Ntb, const int *tb_ind, const float *tb_eff);
+void   o_mix_add_falloff(o_mix *m, double AL, float nL, float EL, double AH, float nH, float EH,
+                         int has_backward, int Ne, const int *ed_ind, const float *ed_nu,
+                         int Np, const int *pd_ind, const float *pd_nu,
+                         int Ntb, const int *tb_ind, const float *tb_eff,
+                         int Ftype, float alpha, double T1, double T2, double T3,
+                         const float *sri5);
+void   o_mix_store_refs(o_mix *m, double PRef, double TRef, double LRef, const double *XiRef);
+
+/* ---- point functions of struct Mix (computational units) ----------------- */
+double o_GetMolarWeightFromYi(const o_mix *m, const double *Yi);
+double o_GetMolarWeightFromXi(const o_mix *m, const double *Xi);
+void   o_GetMolarFractions(const o_mix *m, double *Xi, double MixW, const double *Yi);
+void   o_GetMassFractions(const o_mix *m, double *Yi, double MixW, const double *Xi);
+double o_GetRho(const o_mix *m, double P, double T, double MixW);
+double o_GetHeatCapacity(const o_mix *m, double T, const double *Yi);
+double o_GetEnthalpy(const o_mix *m, double T, const double *Yi);
+double o_GetSpeciesEnthalpy(const o_mix *m, int i, double T);
+double o_GetInternalEnergy(const o_mix *m, double T, const double *Yi);
+double o_GetTFromInternalEnergy(const o_mix *m, double e0, double T, const double *Yi);
+int    o_GetTFromInternalEnergy_iters(const o_mix *m, double e0, double T, const double *Yi);
+double o_GetTFromRhoAndP(const o_mix *m, double rho, double MixW, double P);
+double o_GetPFromRhoAndT(const o_mix *m, double rho, double MixW, double T);
+double o_GetViscosity(const o_mix *m, double T, const double *Xi);
+double o_GetHeatConductivity(const o_mix *m, double T, const double *Xi);
+double o_GetGamma(const o_mix *m, double T, double MixW, const double *Yi);
+double o_GetSpeedOfSound(const o_mix *m, double T, double gamma, double MixW);
+void   o_GetDiffusivity(const o_mix *m, double *Di, double P, double T, double MixW, const double *Xi);
+void   o_GetProductionRates(const o_mix *m, double *w, double rho, double P, double T, const double *Yi);
+double o_Getdpde(const o_mix *m, double rho, double gamma);
+void   o_Getdpdrhoi(const o_mix *m, double *dpdrhoi, double gamma, double T, const double *Yi);
+int    o_isValidInternalEnergy(const o_mix *m, double e, const double *Yi);
+
+/* ---- stencil tables (src/prometeo_metric_coeffs.cc) ---------------------- */
+int    o_coeff_Cp(int nType, int k);
+double o_coeff_ReconPlus(int nType, int k);
+double o_coeff_ReconMinus(int nType, int k);
+double o_coeff_CoeffsPlus(int nType, int k);
+double o_coeff_CoeffsMinus(int nType, int k);
+float  o_coeff_Interp(int nType, int k);
+float  o_coeff_Grad(int nType, int k);
+int    o_coeff_KennedyOrder(int nType);
+double o_coeff_KennedyCoeff(int nType, int k);
+int    o_coeff_KennedyNSum(int nType);
+float  o_coeff_p10(int k);
+
+/* ---- reconstruction operators on 6 scalars ------------------------------ */
+/* op: 0 TENO_Op<-10>, 1 TENOA_Op<-8>, 2 TENOLAD_Op<-16>; side 0 plus, 1 minus.
+ * bits (may be NULL) receives the 4 stencil-selection bits b1..b4 */
+double o_teno_reconstruct(int op, int side, const double *y6, int nType, int *bits);
+int    o_tenoa_sensor(const double *y6, int nType, double Phi);
+double o_kennedy_reconstruct3(const double *rho, const double *u, const double *phi, int nType);
+
+/* ---- tasks (one call = the reference cpu_base_impl over the rect) ------- */
+void   o_update_properties(const o_grid *g, const o_fields *f, const o_mix *m);
+void   o_update_conserved(const o_grid *g, const o_fields *f, const o_mix *m);
+void   o_update_primitive(const o_grid *g, const o_fields *f, const o_mix *m);
+void   o_ducros_sensor(const o_grid *g, const o_fields *f, double vorticityScale);
+void   o_shock_sensor(const o_grid *g, const o_fields *f, int dir, const o_mix *m);
+/* scheme: 0 hybrid, 1 TENO, 2 TENOA, 3 TENOLAD, 4 skew-symmetric (KG).
+ * decisions (may be NULL): per-cell uint32 decision word of the face at the
+ * cell's + side (see o_rhs.c) */
+void   o_euler_flux(const o_grid *g, const o_fields *f, int dir, int scheme,
+                    double RK0, double RK1, double deltaTime, const o_mix *m,
+                    uint32_t *decisions);
+void   o_diffusion_flux(const o_grid *g, const o_fields *f, int dir, const o_mix *m);
+void   o_chem_source(const o_grid *g, const o_fields *f, const o_mix *m);
+/* nsteps (may be NULL): Rosenbrock steps per cell */
+void   o_chem_update(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps);
+double o_max_spectral_radius(const o_grid *g, const o_fields *f, const o_mix *m);
+/* SSP-RK3 update of Conserved (src/prometeo_rk.rg:88-112) */
+void   o_rk_update(const o_grid *g, const o_fields *f, int nEq, int stage, double dt);
+
+/* full RHS of one RK stage in the reference order (src/prometeo.rg:503-578):
+ * zero Conserved_t, chemistry source, Euler Z,Y,X, diffusion Z,Y,X */
+void   o_full_rhs(const o_grid *g, const o_fields *f, int scheme, double RK0, double RK1,
+                  double deltaTime, int chem_source, const o_mix *m);
+
+/* ---- dense linear algebra / implicit solver (src/Utils/math_utils.hpp) -- */
+/* LU with implicit scaling + partial pivoting on a column-major n x n A,
+ * then solve A x = b in place */
+void   o_lu_solve(int n, const double *A, double *b);
+/* reference known-answer problem of CPPmathUtilsTest.cc */
+void   o_rosenbrock_testprob(double *x, double dtTry, double DelT, int32_t *nsteps);
+/* one cell's implicit chemistry: x[nEq] in/out, temperature in/out */
+int    o_chem_cell(const o_mix *m, double *x, const double *Conserved_t_old,
+                   double *temperature, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
