@@ -1,0 +1,570 @@
+// euler.hip -- Euler flux divergence along one direction: hybrid KG/TENO,
+// pure TENO/TENO-A/TENO-LAD and pure Kennedy-Gruber (skew-symmetric).
+//
+// Reference: src/prometeo_rhs.inl:40-1052 (ComputeKGSums, KGFluxReconstruction,
+// ComputeRoeAverages, characteristic projections, FluxReconstruction<Op>,
+// the three updateRHSSpan loops), src/prometeo_rhs.cu:37-459 (GPU variants).
+//
+// The face flux is a function of its left cell only: the reference's
+// Kennedy-Gruber ring buffer holds pair sums S(a,b) of two cells, so
+// recomputing them per face gives the same values, and its "first minus
+// inter-cell location" is the face of the cell before the first one.  Two
+// traversals, both with consecutive lanes on consecutive x cells:
+//   y and z sweeps: a thread walks a segment of SEG cells along the sweep,
+//     carrying FluxM and the KG pair sums as the reference pencil loop does;
+//   x sweep: a wave walks a list of x rows face by face, lane l holding face
+//     base+l; the minus face of a cell is the flux of lane l-1 (wave shuffle),
+//     lane 0 takes it from lane 63 of the previous round.
+// The TENO path evaluates the characteristic projection one component at a
+// time (left-eigenvector rows of rhs.inl:436-505 with the per-point sums
+// hoisted), so no 6 x nEq split-flux arrays are held in registers.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace htr {
+
+template<class MX, int DIR>
+struct Euler {
+   static constexpr int NS = MX::NS, NEQ = MX::NEQ;
+   static constexpr int iN = DIR, iT1 = (DIR == 0) ? 1 : 0, iT2 = (DIR == 2) ? 1 : 2;
+
+   const htr_fields &f;
+   const DMix &m;
+   const Geo &g;
+
+   __device__ double cons(const P3 &p, int i) const { return ld<double>(f.Conserved, p, i); }
+   __device__ double vel(const P3 &p, int i) const { return ld<double>(f.velocity, p, i); }
+
+   // ---- Kennedy-Gruber pair sums S(a,b), b the right cell (rhs.inl:42-75) ----
+   __device__ void pair_sum(double *S, const P3 &a, const P3 &b) const
+   {
+      const double rho_a = ld<double>(f.rho, a), rho_b = ld<double>(f.rho, b);
+      const double pa = ld<double>(f.pressure, a), pb = ld<double>(f.pressure, b);
+      const double H = (cons(a, NS+3) + pa)/rho_a;
+      const double rhom = rho_a + rho_b;
+      const double vm = -(vel(a, iN) + vel(b, iN));
+#pragma unroll
+      for (int i = 0; i < NS; i++) S[i] = rhom*vm*(ld<double>(f.MassFracs, a, i) + ld<double>(f.MassFracs, b, i));
+#pragma unroll
+      for (int i = 0; i < 3; i++) S[NS+i] = rhom*vm*(vel(a, i) + vel(b, i));
+      S[NS+3] = rhom*vm*(H + (cons(b, NS+3) + pb)/rho_b);
+      S[NEQ] = (pa + pb);
+   }
+
+   // staggered-node KG flux (rhs.inl:92-110)
+   __device__ void kg_staggered(double *Flux, const P3 &p, int t) const
+   {
+      const P3 q = (t == L_S_node) ? p : wplus(g, DIR, p, c_st.Cp[t][2]);
+      const double u = vel(q, iN), pr = ld<double>(f.pressure, q);
+#pragma unroll
+      for (int i = 0; i < NEQ; i++) Flux[i] = -(cons(q, i)*u);
+      Flux[NS+iN] -= pr;
+      Flux[NS+3] -= pr*u;
+   }
+
+   // KGFluxReconstruction (rhs.inl:79-135).  The reference reads ring-buffer
+   // slot (5-m)m/2 + l, which holds S(p-m, p-m+l+1); here each pair sum is
+   // formed when it is added, in the reference's order (l outer, m inner).
+   __device__ void kg_flux(double *Flux, const P3 &p, int t) const
+   {
+      if (t == L_S_node || t == Rm1_S_node) { kg_staggered(Flux, p, t); return; }
+      double fl[NEQ+1];
+#pragma unroll
+      for (int i = 0; i < NEQ+1; i++) fl[i] = 0.0;
+      const int order = c_st.KOrder[t];
+      for (int l = 0; l < order; l++) {
+         double acc[NEQ+1];
+#pragma unroll
+         for (int i = 0; i < NEQ+1; i++) acc[i] = 0.0;
+         for (int mm = 0; mm <= l; mm++) {
+            double S[NEQ+1];
+            const P3 a = wminus(g, DIR, p, -mm);
+            pair_sum(S, a, wplus(g, DIR, a, l+1));
+#pragma unroll
+            for (int i = 0; i < NEQ+1; i++) acc[i] += S[i];
+         }
+         const double c = c_st.KCoeff[t][l];
+#pragma unroll
+         for (int i = 0; i < NEQ+1; i++) fl[i] += c*acc[i];
+      }
+#pragma unroll
+      for (int i = 0; i < NEQ; i++) Flux[i] = 0.25*fl[i];
+      Flux[NS+iN] -= fl[NEQ];
+   }
+
+   // ---- Roe state (rhs.inl:200-279) ----------------------------------------
+   struct Roe { double a, e, H, a2, rho, v[3], Y[NS], dpde, dpdrhoi[NS]; };
+
+   __device__ static double mod2(const double *v) { double r = 0.0; r += v[0]*v[0]; r += v[1]*v[1]; r += v[2]*v[2]; return r; }
+
+   __device__ void roe(Roe &R, const P3 &L, const P3 &Rp) const
+   {
+      double vL[3], vR[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) { vL[i] = vel(L, i); vR[i] = vel(Rp, i); }
+      const double rhoL = ld<double>(f.rho, L), rhoR = ld<double>(f.rho, Rp);
+      const double pL = ld<double>(f.pressure, L), pR = ld<double>(f.pressure, Rp);
+      const double IeL = cons(L, NS+3)/rhoL - 0.5*mod2(vL);
+      const double IeR = cons(Rp, NS+3)/rhoR - 0.5*mod2(vR);
+      const double HL = (cons(L, NS+3) + pL)/rhoL;
+      const double HR = (cons(Rp, NS+3) + pR)/rhoR;
+      const double fL = sqrt(rhoL)/(sqrt(rhoL) + sqrt(rhoR));
+      const double fR = sqrt(rhoR)/(sqrt(rhoL) + sqrt(rhoR));
+      R.rho = sqrt(rhoL*rhoR);
+#pragma unroll
+      for (int i = 0; i < NS; i++) R.Y[i] = ld<double>(f.MassFracs, L, i)*fL + ld<double>(f.MassFracs, Rp, i)*fR;
+#pragma unroll
+      for (int i = 0; i < 3; i++) R.v[i] = vL[i]*fL + vR[i]*fR;
+      R.H = HL*fL + HR*fR;
+      R.e = IeL*fL + IeR*fR;
+      const double PovRho = R.H - R.e - 0.5*mod2(R.v);
+      const double MixW = MX::MolarWeightFromY(m, R.Y);
+      const double Teos = MX::TFromRhoAndP(m, R.rho, MixW, PovRho*R.rho);
+      const double T = MX::TFromInternalEnergy(m, R.e, Teos, R.Y);
+      const double gam = MX::Gamma(m, T, MixW, R.Y);
+      R.dpde = MX::dpde(R.rho, gam);
+      MX::dpdrhoi(m, R.dpdrhoi, gam, T, R.Y);
+      const double dp = pR - pL;
+      const double de = IeR - IeL;
+      double dpError = dp - de*R.dpde;
+      double fact = de*R.dpde; fact *= fact;
+      double drhoi[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+         drhoi[i] = rhoR*ld<double>(f.MassFracs, Rp, i) - rhoL*ld<double>(f.MassFracs, L, i);
+         const double dpi = drhoi[i]*R.dpdrhoi[i];
+         dpError -= dpi;
+         fact += dpi*dpi;
+      }
+      fact = dpError/dmax(fact, 1e-6);
+#pragma unroll
+      for (int i = 0; i < NS; i++) R.dpdrhoi[i] = R.dpdrhoi[i]*(1.0 + R.dpdrhoi[i]*drhoi[i]*fact);
+      R.dpde = R.dpde*(1.0 + R.dpde*de*fact);
+      R.a2 = PovRho/R.rho*R.dpde;
+#pragma unroll
+      for (int i = 0; i < NS; i++) R.a2 += R.Y[i]*R.dpdrhoi[i];
+      R.a = sqrt(R.a2);
+   }
+
+   // Constants of the left-eigenvector rows (rhs.inl:443-455)
+   struct Proj { double iaRoe, Coeff, d, cv[3], iaRoe2; };
+
+   // Row i of projectToCharacteristicSpace applied to the vector w, with
+   // sum1 = sum_j (Coeff - dpdrhoi_j) iaRoe2 w_j and sumQsp = sum_j w_j
+   // precomputed; wm[0..3] are w[NS..NS+3] and ws = w[i-1] for species rows.
+   __device__ static double char_row(int i, const Roe &R, const Proj &P, double sum1, double sumQsp,
+                                     double ws, const double *wm)
+   {
+      if (i == 0 || i == NEQ-1) {
+         const double sg = (i == 0) ? 1.0 : -1.0;
+         double r = (i == 0) ? sumQsp*R.v[iN]*P.iaRoe - sum1 : -sumQsp*R.v[iN]*P.iaRoe - sum1;
+         r -= (P.cv[iN] + sg*P.iaRoe)*wm[iN];
+         r -= P.cv[iT1]*wm[iT1];
+         r -= P.cv[iT2]*wm[iT2];
+         r += P.d*wm[3];
+         return r*0.5;
+      }
+      if (i <= NS) {
+         const double Y = R.Y[i-1];
+         double r = Y*sum1;
+         r += ws;
+         r += Y*P.cv[0]*wm[0];
+         r += Y*P.cv[1]*wm[1];
+         r += Y*P.cv[2]*wm[2];
+         r += -Y*P.d*wm[3];
+         return r;
+      }
+      if (i == NS+1) { double r = -R.v[iT1]*sumQsp; r += wm[iT1]; return r; }
+      double r = -R.v[iT2]*sumQsp; r += wm[iT2]; return r;
+   }
+
+   // Column i of projectFromCharacteristicSpace (rhs.inl:510-572) applied to
+   // the scalar q and added to r.  Called for i = 0..NEQ-1 in order, it sums
+   // every component in the reference's order: sumQsp collects the species
+   // rows and enters the momentum rows where the reference adds it.
+   __device__ static void add_column(double *r, double &sumQsp, int i, double q, const Roe &R,
+                                     double ERoe, double dedp)
+   {
+      if (i == 0) {
+#pragma unroll
+         for (int s = 0; s < NS; s++) r[s] = R.Y[s]*q;
+         r[NS+iN] = (R.v[iN] - R.a)*q;
+         r[NS+iT1] = R.v[iT1]*q;
+         r[NS+iT2] = R.v[iT2]*q;
+         r[NS+3] = (R.H - R.v[iN]*R.a)*q;
+         sumQsp = 0.0;
+      } else if (i <= NS) {
+#pragma unroll
+         for (int s = 0; s < NS; s++) if (s == i-1) {
+            r[s] += q;
+            r[NS+3] += (ERoe - R.rho*R.dpdrhoi[s]*dedp)*q;
+         }
+         sumQsp += q;
+      } else if (i == NS+1) {
+         r[NS+iN] += R.v[iN]*sumQsp;
+         r[NS+iT1] += R.v[iT1]*sumQsp;
+         r[NS+iT1] += q;
+         r[NS+3] += R.v[iT1]*q;
+      } else if (i == NS+2) {
+         r[NS+iT2] += R.v[iT2]*sumQsp;
+         r[NS+iT2] += q;
+         r[NS+3] += R.v[iT2]*q;
+      } else {
+#pragma unroll
+         for (int s = 0; s < NS; s++) r[s] += R.Y[s]*q;
+         r[NS+iN] += (R.v[iN] + R.a)*q;
+         r[NS+iT1] += R.v[iT1]*q;
+         r[NS+iT2] += R.v[iT2]*q;
+         r[NS+3] += (R.H + R.v[iN]*R.a)*q;
+      }
+   }
+
+   // ± split flux of component i at point k (getPlusMinusFlux, rhs.inl:586-620)
+   __device__ void split(double &vp, double &vm, int i, const P3 &P, double un, double pr, double lam,
+                         const Roe &R, const Proj &J, double s1q, double sqq, double s1f, double sqf) const
+   {
+      double qm[4], fm[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) { qm[j] = cons(P, NS+j); fm[j] = qm[j]*un; }
+      fm[iN] += pr;
+      fm[3] += pr*un;
+      const double qs = (i >= 1 && i <= NS) ? cons(P, i-1) : 0.0;
+      const double Q = char_row(i, R, J, s1q, sqq, qs, qm);
+      const double F = char_row(i, R, J, s1f, sqf, qs*un, fm);
+      vp = F + lam*Q;
+      vm = F - lam*Q;
+   }
+
+   // FluxReconstruction<Op> (rhs.inl:625-759), one characteristic component
+   // at a time: the TENO result of component i goes straight into the
+   // back-projection, so no per-point split-flux vectors are held.
+   template<int OP>
+   __device__ void teno_flux(double *Flux, const P3 &p, int t, double RK0, double RK1,
+                             double lim_f, uint32_t *dec) const
+   {
+      P3 P[6];
+      P[0] = wminus(g, DIR, p, c_st.Cp[t][0]);
+      P[1] = wminus(g, DIR, p, c_st.Cp[t][1]);
+      P[2] = p;
+      P[3] = wplus(g, DIR, p, c_st.Cp[t][2]);
+      P[4] = wplus(g, DIR, p, c_st.Cp[t][3]);
+      P[5] = wplus(g, DIR, p, c_st.Cp[t][4]);
+      double un[6], pr[6];
+      double Lam1, Lam, LamN;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+         un[k] = vel(P[k], iN);
+         pr[k] = ld<double>(f.pressure, P[k]);
+         const double c = ld<double>(f.SoS, P[k]);
+         const double l1 = fabs(un[k] - c), l = fabs(un[k]), lN = fabs(un[k] + c);
+         if (k == 0) { Lam1 = l1; Lam = l; LamN = lN; }
+         else { Lam1 = dmax(Lam1, l1); Lam = dmax(Lam, l); LamN = dmax(LamN, lN); }
+      }
+      Roe R;
+      roe(R, p, P[3]);
+      Proj J;
+      J.iaRoe = 1.0/R.a;
+      J.iaRoe2 = 1.0/R.a2;
+      J.Coeff = (R.e - 0.5*mod2(R.v))*R.dpde/R.rho;
+      J.d = R.dpde/(R.rho*R.a2);
+#pragma unroll
+      for (int j = 0; j < 3; j++) J.cv[j] = R.v[j]*J.d;
+      const double dedp = 1.0/R.dpde;
+      const double ERoe = R.e + 0.5*mod2(R.v);
+      // per-point species sums of the conserved vector (q) and the Euler flux (f)
+      double s1q[6], sqq[6], s1f[6], sqf[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+         double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+         for (int j = 0; j < NS; j++) {
+            const double q = cons(P[k], j);
+            const double w = (J.Coeff - R.dpdrhoi[j])*J.iaRoe2;
+            a += w*q;
+            b += q;
+            const double fq = q*un[k];
+            c += w*fq;
+            d += fq;
+         }
+         s1q[k] = a; sqq[k] = b; s1f[k] = c; sqf[k] = d;
+      }
+      uint32_t h = 0;
+      double sumQsp = 0.0;
+#pragma unroll 1
+      for (int i = 0; i < NEQ; i++) {
+         const double lam = (i == 0) ? Lam1 : (i == NEQ-1) ? LamN : Lam;
+         double vp[6], vm[6];
+#pragma unroll
+         for (int k = 0; k < 6; k++) split(vp[k], vm[k], i, P[k], un[k], pr[k], lam, R, J, s1q[k], sqq[k], s1f[k], sqf[k]);
+         uint32_t bp, bm;
+         const double FPlus  = teno_reconstruct<OP, 0>(vp, t, &bp);
+         const double FMinus = teno_reconstruct<OP, 1>(vm, t, &bm);
+         h = fold_bits(fold_bits(h, bp), bm);
+         add_column(Flux, sumQsp, i, -0.5*(FPlus + FMinus), R, ERoe, dedp);
+      }
+      uint32_t lim_bit = 0;
+      if (NS > 1) {
+         bool limit = false;
+#pragma unroll
+         for (int i = 0; i < NS; i++) {
+            const double dRhoi = lim_f*Flux[i];
+            if (((RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i) + dRhoi) < -1e-8) ||
+                ((RK0*ld<double>(f.Conserved_old, P[3], i) + RK1*cons(P[3], i) - dRhoi) < -1e-8))
+               limit = true;
+         }
+         if (limit) {
+            lim_bit = 2;
+            // first-order flux -0.5*(FluxP(p) + FluxM(p+1)), back-projected
+            double FLO[NEQ];
+            double sq = 0.0;
+#pragma unroll 1
+            for (int i = 0; i < NEQ; i++) {
+               const double lam = (i == 0) ? Lam1 : (i == NEQ-1) ? LamN : Lam;
+               double ap, am, bp, bm;
+               split(ap, am, i, P[2], un[2], pr[2], lam, R, J, s1q[2], sqq[2], s1f[2], sqf[2]);
+               split(bp, bm, i, P[3], un[3], pr[3], lam, R, J, s1q[3], sqq[3], s1f[3], sqf[3]);
+               add_column(FLO, sq, i, -0.5*(ap + bm), R, ERoe, dedp);
+            }
+            const double eps = 1e-60;
+            double theta = 1.0;
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+               const double dRhoi = lim_f*Flux[i];
+               const double dRhoi_lo = lim_f*FLO[i];
+               double theta_p = 1.0;
+               const double rhoi_p = RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i);
+               const double rhoHO_p = rhoi_p + dRhoi;
+               if (rhoHO_p < eps) {
+                  const double rhoLO = rhoi_p + dRhoi_lo;
+                  theta_p = (fabs(rhoHO_p - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_p - rhoLO)) : 1.0;
+               }
+               double theta_pp1 = 1.0;
+               const double rhoi_pp1 = RK0*ld<double>(f.Conserved_old, P[3], i) + RK1*cons(P[3], i);
+               const double rhoHO_pp1 = rhoi_pp1 - dRhoi;
+               if (rhoHO_pp1 < eps) {
+                  const double rhoLO = rhoi_pp1 - dRhoi_lo;
+                  theta_pp1 = (fabs(rhoHO_pp1 - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_pp1 - rhoLO)) : 1.0;
+               }
+               theta = dmin(dmin(theta_p, theta_pp1), theta);
+            }
+#pragma unroll
+            for (int i = 0; i < NEQ; i++) Flux[i] = Flux[i]*theta + FLO[i]*(1 - theta);
+         }
+      }
+      *dec = (h << 2) | lim_bit;
+   }
+
+   // TENO-family face flux for the scheme
+   template<int SCHEME>
+   __device__ void teno_face(double *Flux, const P3 &p, int t, double RK0, double RK1, double dt,
+                             uint32_t *dec) const
+   {
+      const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
+      const double lim_f = dt*(ld<double>(f.m_e[DIR], p) + ld<double>(f.m_e[DIR], pP1));
+      constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
+      teno_flux<OP>(Flux, p, t, RK0, RK1, lim_f, dec);
+   }
+
+   __device__ bool sensor_kg(const P3 &p, int t) const
+   {
+      const P3 pM1 = wminus(g, DIR, p, c_st.Cp[t][1]);
+      const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
+      return ld<uint8_t>(f.shockSensor[DIR], pM1) && ld<uint8_t>(f.shockSensor[DIR], p) &&
+             ld<uint8_t>(f.shockSensor[DIR], pP1);
+   }
+
+   // complete face flux (no ring buffer) for the scheme
+   template<int SCHEME>
+   __device__ void face(double *Flux, const P3 &p, double RK0, double RK1, double dt, uint32_t *dec) const
+   {
+      const int t = ld<int>(f.nType[DIR], p);
+      if (SCHEME == HTR_EULER_KG || (SCHEME == HTR_EULER_HYBRID && sensor_kg(p, t))) {
+         kg_flux(Flux, p, t);
+         *dec = 1;
+         return;
+      }
+      teno_face<SCHEME>(Flux, p, t, RK0, RK1, dt, dec);
+   }
+};
+
+// ---- face enumeration --------------------------------------------------------
+// A sweep along DIR over the execution rect has (esz[DIR] + 1) faces per line:
+// face j = 0 is the minus face of the first cell, taken at the cell
+// p0 + offM1(nType[p0]) as the reference primes FluxM (rhs.inl:800-817), and
+// face j >= 1 is the plus face of cell elo[DIR] + j - 1.  Face ids put the
+// x index fastest, so consecutive lanes touch consecutive cells: for the x
+// sweep id = line*(esz0 + 1) + j, for y and z id = j*nlines + line with the
+// line index x-fastest.
+template<int DIR>
+struct Faces {
+   static constexpr int A = (DIR == 0) ? 1 : 0;     // fastest cross direction
+   static constexpr int B = (DIR == 2) ? 1 : 2;
+   __device__ static void split(const Geo &g, uint32_t fid, uint32_t &line, int &j)
+   {
+      if (DIR == 0) { const uint32_t n = (uint32_t)g.esz[0] + 1; line = fid/n; j = (int)(fid - line*n); }
+      else { const uint32_t nl = (uint32_t)g.esz[A]*(uint32_t)g.esz[B]; j = (int)(fid/nl); line = fid - (uint32_t)j*nl; }
+   }
+   __device__ static uint32_t id(const Geo &g, uint32_t line, int j)
+   {
+      if (DIR == 0) return line*((uint32_t)g.esz[0] + 1) + (uint32_t)j;
+      return (uint32_t)j*((uint32_t)g.esz[A]*(uint32_t)g.esz[B]) + line;
+   }
+   // the cell whose plus face is face fid
+   __device__ static P3 cell(const Geo &g, const htr_fields &f, uint32_t fid)
+   {
+      uint32_t line; int j;
+      split(g, fid, line, j);
+      int c[3];
+      c[A] = g.elo[A] + (int)(line % (uint32_t)g.esz[A]);
+      c[B] = g.elo[B] + (int)(line / (uint32_t)g.esz[A]);
+      c[DIR] = g.elo[DIR] + (j > 0 ? j - 1 : 0);
+      P3 p = mkp(g, c[0], c[1], c[2]);
+      if (j == 0) p = wminus(g, DIR, p, c_st.Cp[ld<int>(f.nType[DIR], p)][1]);
+      return p;
+   }
+   __host__ __device__ static int64_t count(const Geo &g) { return ((int64_t)g.esz[DIR] + 1)*g.esz[A]*g.esz[B]; }
+};
+
+template<int NEQ>
+__device__ inline void store_face(double *__restrict__ F, int64_t nfaces, uint32_t fid, const double *v)
+{
+#pragma unroll
+   for (int i = 0; i < NEQ; i++) F[i*nfaces + fid] = v[i];
+}
+
+// Pass 1: hybrid sensor and Kennedy-Gruber faces; faces that need TENO are
+// appended to a compacted list (one atomic per wave).  With ALL_KG every
+// face is a KG face (UpdateUsingSkewSymmetricEulerFluxTask).
+template<class MX, int DIR, bool ALL_KG>
+__global__ void __launch_bounds__(256)
+euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ F,
+                int64_t nfaces, int32_t *__restrict__ list, int *__restrict__ counter, htr_view dec)
+{
+   const int64_t fid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+   const bool active = fid < nfaces;
+   bool teno = false;
+   if (active) {
+      const P3 q = Faces<DIR>::cell(g, f, (uint32_t)fid);
+      const int t = ld<int>(f.nType[DIR], q);
+      Euler<MX, DIR> E{f, *mix, g};
+      if (ALL_KG || E.sensor_kg(q, t)) {
+         double Flux[MX::NEQ];
+         E.kg_flux(Flux, q, t);
+         store_face<MX::NEQ>(F, nfaces, (uint32_t)fid, Flux);
+         if (dec.ptr) at<uint32_t>(dec, q) = 1;
+      } else {
+         teno = true;
+      }
+   }
+   if (!ALL_KG) {
+      const uint64_t mask = __ballot(teno);
+      if (mask) {
+         const int lane = threadIdx.x & 63;
+         const int leader = __ffsll((unsigned long long)mask) - 1;
+         int base = 0;
+         if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+         base = __shfl(base, leader, 64);
+         if (teno) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)fid;
+      }
+   }
+}
+
+// Pass 2: TENO-family faces, either all faces or the compacted list.
+template<class MX, int DIR, int SCHEME, bool LISTED>
+__global__ void __launch_bounds__(256)
+euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double RK0, double RK1,
+                  double dt, double *__restrict__ F, int64_t nfaces, const int32_t *__restrict__ list,
+                  const int *__restrict__ counter, htr_view dec)
+{
+   const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
+   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
+   Euler<MX, DIR> E{f, *mix, g};
+   for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < n; w += stride) {
+      const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
+      const P3 q = Faces<DIR>::cell(g, f, fid);
+      const int t = ld<int>(f.nType[DIR], q);
+      double Flux[MX::NEQ];
+      uint32_t word;
+      E.template teno_face<SCHEME>(Flux, q, t, RK0, RK1, dt, &word);
+      store_face<MX::NEQ>(F, nfaces, fid, Flux);
+      if (dec.ptr) at<uint32_t>(dec, q) = word;
+   }
+}
+
+// Pass 3: Conserved_t[p] += (F(p) - F(p - 1))*m_e[p] (rhs.inl:876-887), one
+// lane per cell of the execution rect.
+template<int NEQ, int DIR>
+__global__ void __launch_bounds__(256)
+face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict__ F, int64_t nfaces,
+                       const htr_view metric, int64_t ncells)
+{
+   const int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+   if (i >= ncells) return;
+   const P3 p = exec_point(g, i);
+   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B;
+   const uint32_t line = (uint32_t)(p.c[A] - g.elo[A]) + (uint32_t)(p.c[B] - g.elo[B])*(uint32_t)g.esz[A];
+   const int j = p.c[DIR] - g.elo[DIR] + 1;
+   const uint32_t fp = Faces<DIR>::id(g, line, j);
+   const uint32_t fm = Faces<DIR>::id(g, line, j - 1);
+   const double m = ld<double>(metric, p);
+#pragma unroll
+   for (int c = 0; c < NEQ; c++)
+      at<double>(f.Conserved_t, p, c) += (F[c*nfaces + fp] - F[c*nfaces + fm])*m;
+}
+
+template<class MX, int DIR, int SCHEME>
+static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
+                      double dt, const htr_view &dec, hipStream_t s)
+{
+   const int64_t ncells = exec_cells(g);
+   if (ncells == 0) return HTR_OK;
+   const int64_t nfaces = Faces<DIR>::count(g);
+   if (nfaces >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many faces");
+   Workspace &w = workspace();
+   int rc = workspace_faces(w, nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0);
+   if (rc) return rc;
+   constexpr int kMaxTenoBlocks = 256*8;
+   if constexpr (SCHEME == HTR_EULER_KG) {
+      hipLaunchKernelGGL((euler_kg_kernel<MX, DIR, true>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
+                         f, mix, g, w.faces, nfaces, w.list, w.counter + 1, dec);
+      if ((rc = check_launch("euler_kg_kernel"))) return rc;
+   } else if constexpr (SCHEME == HTR_EULER_HYBRID) {
+      if ((rc = zero_counters(w.counter, s))) return rc;
+      hipLaunchKernelGGL((euler_kg_kernel<MX, DIR, false>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
+                         f, mix, g, w.faces, nfaces, w.list, w.counter + 1, dec);
+      if ((rc = check_launch("euler_kg_kernel"))) return rc;
+      const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxTenoBlocks);
+      hipLaunchKernelGGL((euler_teno_kernel<MX, DIR, SCHEME, true>), dim3(nb), dim3(256), 0, s,
+                         f, mix, g, RK0, RK1, dt, w.faces, nfaces, w.list, w.counter, dec);
+      if ((rc = check_launch("euler_teno_kernel"))) return rc;
+   } else {
+      const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxTenoBlocks);
+      hipLaunchKernelGGL((euler_teno_kernel<MX, DIR, SCHEME, false>), dim3(nb), dim3(256), 0, s,
+                         f, mix, g, RK0, RK1, dt, w.faces, nfaces, w.list, w.counter, dec);
+      if ((rc = check_launch("euler_teno_kernel"))) return rc;
+   }
+   hipLaunchKernelGGL((face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)), dim3(256), 0, s,
+                      f, g, w.faces, nfaces, f.m_e[DIR], ncells);
+   return check_launch("face_divergence_kernel");
+}
+
+template<class MX>
+int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
+                   double RK0, double RK1, double dt, const htr_view &dec, hipStream_t s)
+{
+#define HTR_E_CASE(D, S) if (dir == D && scheme == S) return launch_dir<MX, D, S>(f, mix, g, RK0, RK1, dt, dec, s);
+#define HTR_E_DIRS(S) HTR_E_CASE(0, S) HTR_E_CASE(1, S) HTR_E_CASE(2, S)
+   HTR_E_DIRS(HTR_EULER_HYBRID)
+   HTR_E_DIRS(HTR_EULER_TENO)
+   HTR_E_DIRS(HTR_EULER_TENOA)
+   HTR_E_DIRS(HTR_EULER_TENOLAD)
+   HTR_E_DIRS(HTR_EULER_KG)
+#undef HTR_E_DIRS
+#undef HTR_E_CASE
+   return set_error(HTR_E_ARG, "euler: bad direction or scheme");
+}
+
+template int euler_dispatch<MixCP>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, double, const htr_view &, hipStream_t);
+template int euler_dispatch<MixAir>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, double, const htr_view &, hipStream_t);
+
+} // namespace htr

@@ -1,0 +1,357 @@
+// pointwise.hip -- the cell-local tasks of the hot path and the spectral
+// radius reduction.
+//
+//   UpdatePropertiesFromPrimitive / UpdateConservedFromPrimitive /
+//   UpdatePrimitiveFromConserved   src/prometeo_variables.hpp:62-192
+//   UpdateDucrosSensor             src/prometeo_sensor.inl:52-77, sensor.cc:37-80
+//   UpdateShockSensor<dir>         src/prometeo_sensor.cc:82-136
+//   AddChemistrySources            src/prometeo_chem.cc:78-112
+//   CalculateMaxSpectralRadius     src/prometeo_cfl.hpp:73-178 (wave64 butterfly
+//                                  max, one partial per block, one-block tail)
+//   RK stage update                src/prometeo_rk.rg:48-112
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace htr {
+
+#define HTR_POINT_LOOP(g, n) \
+   const int64_t tid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; \
+   if (tid >= (n)) return; \
+   const P3 p = exec_point(g, tid);
+
+// ------------------------------------------------------------------ variables
+template<class MX>
+__global__ void __launch_bounds__(256)
+properties_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NS = MX::NS;
+   const DMix &m = *mix;
+   double X[NS], Y[NS], D[NS];
+#pragma unroll
+   for (int i = 0; i < NS; i++) X[i] = ld<double>(f.MolarFracs, p, i);
+   const double P = ld<double>(f.pressure, p), T = ld<double>(f.temperature, p);
+   const double MixW = MX::MolarWeightFromX(m, X);
+   at<double>(f.rho, p) = MX::Rho(m, P, T, MixW);
+   at<double>(f.mu, p) = MX::Viscosity(m, T, X);
+   at<double>(f.lam, p) = MX::HeatConductivity(m, T, X);
+   MX::Diffusivity(m, D, P, T, MixW, X);
+   MX::MassFractions(m, Y, MixW, X);
+#pragma unroll
+   for (int i = 0; i < NS; i++) { at<double>(f.Di, p, i) = D[i]; at<double>(f.MassFracs, p, i) = Y[i]; }
+   const double gamma = MX::Gamma(m, T, MixW, Y);
+   at<double>(f.SoS, p) = MX::SpeedOfSound(m, T, gamma, MixW);
+}
+
+template<class MX>
+__global__ void __launch_bounds__(256)
+conserved_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NS = MX::NS;
+   const DMix &m = *mix;
+   double Y[NS], v[3];
+   const double rho = ld<double>(f.rho, p);
+#pragma unroll
+   for (int i = 0; i < NS; i++) { Y[i] = ld<double>(f.MassFracs, p, i); at<double>(f.Conserved, p, i) = rho*Y[i]; }
+#pragma unroll
+   for (int i = 0; i < 3; i++) { v[i] = ld<double>(f.velocity, p, i); at<double>(f.Conserved, p, NS+i) = rho*v[i]; }
+   double k2 = 0.0; k2 += v[0]*v[0]; k2 += v[1]*v[1]; k2 += v[2]*v[2];
+   at<double>(f.Conserved, p, NS+3) = rho*(0.5*k2 + MX::InternalEnergy(m, ld<double>(f.temperature, p), Y));
+}
+
+template<class MX>
+__global__ void __launch_bounds__(256)
+primitive_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NS = MX::NS;
+   const DMix &m = *mix;
+   double rhoYi[NS], Yi[NS], X[NS], v[3];
+#pragma unroll
+   for (int i = 0; i < NS; i++) rhoYi[i] = ld<double>(f.Conserved, p, i);
+   double rho;
+   if (MX::CP) {
+      rho = rhoYi[0];
+      Yi[0] = rhoYi[0]/rho;
+   } else {
+      rho = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) rho += rhoYi[i];
+      const double irho = 1.0/rho;
+#pragma unroll
+      for (int i = 0; i < NS; i++) { Yi[i] = irho*rhoYi[i]; Yi[i] = dmax(Yi[i], 1e-60); Yi[i] = dmin(Yi[i], 1.0); }
+   }
+   const double MixW = MX::MolarWeightFromY(m, Yi);
+   MX::MolarFractions(m, X, MixW, Yi);
+   const double rhoInv = 1.0/rho;
+#pragma unroll
+   for (int i = 0; i < 3; i++) v[i] = ld<double>(f.Conserved, p, NS+i)*rhoInv;
+   double k2 = 0.0; k2 += v[0]*v[0]; k2 += v[1]*v[1]; k2 += v[2]*v[2];
+   const double e = ld<double>(f.Conserved, p, NS+3)*rhoInv - 0.5*k2;
+   const double T = MX::TFromInternalEnergy(m, e, ld<double>(f.temperature, p), Yi);
+   at<double>(f.temperature, p) = T;
+   at<double>(f.pressure, p) = MX::PFromRhoAndT(m, rho, MixW, T);
+#pragma unroll
+   for (int i = 0; i < NS; i++) at<double>(f.MolarFracs, p, i) = X[i];
+#pragma unroll
+   for (int i = 0; i < 3; i++) at<double>(f.velocity, p, i) = v[i];
+}
+
+// -------------------------------------------------------------------- sensors
+template<int DD>
+__device__ inline void vgrad(const htr_fields &f, const Geo &g, const P3 &p, double *out)
+{
+   const int t = ld<int>(f.nType[DD], p);
+   const double mm = ld<double>(f.m_d[DD], p);
+   const P3 a = wminus(g, DD, p, c_st.Cp[t][1]);
+   const P3 b = wplus(g, DD, p, c_st.Cp[t][2]);
+   const double g0 = (double)c_st.Grad[t][0], g1 = (double)c_st.Grad[t][1];
+#pragma unroll
+   for (int i = 0; i < 3; i++) {
+      const double x = ld<double>(f.velocity, p, i);
+      out[i] = (g0*(x - ld<double>(f.velocity, a, i)) + g1*(ld<double>(f.velocity, b, i) - x))*mm;
+   }
+}
+
+__global__ void __launch_bounds__(256)
+ducros_kernel(const htr_fields f, const Geo g, double eps, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   double gx[3], gy[3], gz[3];
+   vgrad<0>(f, g, p, gx);
+   vgrad<1>(f, g, p, gy);
+   vgrad<2>(f, g, p, gz);
+   const double div = gx[0] + gy[1] + gz[2];
+   const double div2 = div*div;
+   const double omz = gx[1] - gy[0];
+   const double omy = gy[2] - gz[1];
+   const double omx = gz[0] - gx[2];
+   const double om2 = omx*omx + omy*omy + omz*omz;
+   at<double>(f.DucrosSensor, p) = div2/(div2 + om2 + eps);
+}
+
+template<class MX, int DIR>
+__global__ void __launch_bounds__(256)
+shock_sensor_kernel(const htr_fields f, const Geo g, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   const int t = ld<int>(f.nType[DIR], p);
+   P3 q[6];
+   q[0] = wminus(g, DIR, p, c_st.Cp[t][0]);
+   q[1] = wminus(g, DIR, p, c_st.Cp[t][1]);
+   q[2] = p;
+   q[3] = wplus(g, DIR, p, c_st.Cp[t][2]);
+   q[4] = wplus(g, DIR, p, c_st.Cp[t][3]);
+   q[5] = wplus(g, DIR, p, c_st.Cp[t][4]);
+   double Phi = ld<double>(f.DucrosSensor, q[0]);
+#pragma unroll
+   for (int k = 1; k < 6; k++) Phi = dmax(Phi, ld<double>(f.DucrosSensor, q[k]));
+   bool sensor = true;
+   for (int h = 0; h < MX::NS && sensor; h++) {
+      double y[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) y[k] = ld<double>(f.Conserved, q[k], h);
+      sensor = tenoa_sensor(y, t, Phi);
+   }
+   at<uint8_t>(f.shockSensor[DIR], p) = sensor ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ chemistry
+template<class MX>
+__global__ void __launch_bounds__(256)
+chem_source_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NS = MX::NS;
+   double Y[NS], w[NS];
+#pragma unroll
+   for (int i = 0; i < NS; i++) Y[i] = ld<double>(f.MassFracs, p, i);
+   MX::ProductionRates(*mix, w, ld<double>(f.rho, p), ld<double>(f.pressure, p), ld<double>(f.temperature, p), Y);
+#pragma unroll
+   for (int i = 0; i < NS; i++) at<double>(f.Conserved_t, p, i) += w[i];
+}
+
+// ----------------------------------------------------------- spectral radius
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, 64));
+   return v;
+}
+
+template<class MX>
+__global__ void __launch_bounds__(256)
+spectral_radius_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n,
+                       double *__restrict__ partial)
+{
+   const int64_t tid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+   double r = 0.0;
+   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
+   for (int64_t i = tid; i < n; i += stride) {
+      const P3 p = exec_point(g, i);
+      const double dcsi = ld<double>(f.m_d[0], p), deta = ld<double>(f.m_d[1], p), dzet = ld<double>(f.m_d[2], p);
+      const double SoS = ld<double>(f.SoS, p);
+      double rr = (fabs(ld<double>(f.velocity, p, 0)) + SoS)*dcsi;
+      rr = dmax(rr, (fabs(ld<double>(f.velocity, p, 1)) + SoS)*deta);
+      rr = dmax(rr, (fabs(ld<double>(f.velocity, p, 2)) + SoS)*dzet);
+      const double rho = ld<double>(f.rho, p);
+      const double nu = ld<double>(f.mu, p)/rho;
+      double rv = nu*dcsi*dcsi;
+      rv = dmax(rv, nu*deta*deta);
+      rv = dmax(rv, nu*dzet*dzet);
+      rv *= 4;
+      rr = dmax(rr, rv);
+      double Y[MX::NS];
+#pragma unroll
+      for (int s = 0; s < MX::NS; s++) Y[s] = ld<double>(f.MassFracs, p, s);
+      const double cp = MX::HeatCapacity(*mix, ld<double>(f.temperature, p), Y);
+      const double DifT = ld<double>(f.lam, p)/(cp*rho);
+      double rt = DifT*dcsi*dcsi;
+      rt = dmax(rt, DifT*deta*deta);
+      rt = dmax(rt, DifT*dzet*dzet);
+      rt *= 4;
+      rr = dmax(rr, rt);
+      double rs = 0.0;
+#pragma unroll
+      for (int s = 0; s < MX::NS; s++) {
+         const double D = ld<double>(f.Di, p, s);
+         rs = dmax(rs, D*dcsi*dcsi);
+         rs = dmax(rs, D*deta*deta);
+         rs = dmax(rs, D*dzet*dzet);
+      }
+      rs *= 4;
+      rr = dmax(rr, rs);
+      if (rr > r) r = rr;
+   }
+   __shared__ double sm[4];
+   r = wave_max(r);
+   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+   __syncthreads();
+   if (threadIdx.x == 0) partial[blockIdx.x] = dmax(dmax(sm[0], sm[1]), dmax(sm[2], sm[3]));
+}
+
+__global__ void __launch_bounds__(256)
+reduce_max_kernel(const double *__restrict__ partial, int n, double *__restrict__ out)
+{
+   double r = 0.0;
+   for (int i = threadIdx.x; i < n; i += blockDim.x) r = dmax(r, partial[i]);
+   __shared__ double sm[4];
+   r = wave_max(r);
+   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+   __syncthreads();
+   if (threadIdx.x == 0) *out = dmax(dmax(sm[0], sm[1]), dmax(sm[2], sm[3]));
+}
+
+// ------------------------------------------------------------------------ RK
+__global__ void __launch_bounds__(256)
+rk_update_kernel(const htr_fields f, const Geo g, int nEq, double c0, double c1, double c2,
+                 double dt, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   for (int i = 0; i < nEq; i++)
+      at<double>(f.Conserved, p, i) = c0*ld<double>(f.Conserved_old, p, i) + c1*ld<double>(f.Conserved, p, i)
+                                    + c2*ld<double>(f.Conserved_t, p, i)*dt;
+}
+
+__global__ void __launch_bounds__(256)
+rk_init_kernel(const htr_fields f, const Geo g, int nEq, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   for (int i = 0; i < nEq; i++) at<double>(f.Conserved_old, p, i) = ld<double>(f.Conserved, p, i);
+}
+
+__global__ void __launch_bounds__(256)
+zero_rhs_kernel(const htr_fields f, const Geo g, int nEq, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   for (int i = 0; i < nEq; i++) at<double>(f.Conserved_t, p, i) = 0.0;
+}
+
+// clears the device counters (a kernel rather than a memset, so the call
+// sequence stays valid inside a captured graph)
+__global__ void zero_counters_kernel(int *counter)
+{
+   if (threadIdx.x < 4) counter[threadIdx.x] = 0;
+}
+
+int zero_counters(int *counter, hipStream_t s)
+{
+   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
+   return check_launch("zero_counters_kernel");
+}
+
+// ------------------------------------------------------------------ launchers
+#define HTR_LAUNCH_POINT(kern, ...)                                                       \
+   const int64_t n = exec_cells(g);                                                       \
+   if (n == 0) return HTR_OK;                                                             \
+   hipLaunchKernelGGL(kern, dim3(grid_for(n, 256)), dim3(256), 0, s, __VA_ARGS__, n);     \
+   return check_launch(#kern);
+
+template<class MX> int properties_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
+{ HTR_LAUNCH_POINT(properties_kernel<MX>, f, mix, g) }
+template<class MX> int conserved_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
+{ HTR_LAUNCH_POINT(conserved_kernel<MX>, f, mix, g) }
+template<class MX> int primitive_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
+{ HTR_LAUNCH_POINT(primitive_kernel<MX>, f, mix, g) }
+template<class MX> int chem_source_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
+{ HTR_LAUNCH_POINT(chem_source_kernel<MX>, f, mix, g) }
+
+int ducros_launch(const htr_fields &f, const Geo &g, double vorticityScale, hipStream_t s)
+{
+   const double eps = dmax(vorticityScale*vorticityScale, 1e-6);
+   HTR_LAUNCH_POINT(ducros_kernel, f, g, eps)
+}
+
+template<class MX> int shock_sensor_launch(int dir, const htr_fields &f, const Geo &g, hipStream_t s)
+{
+   if (dir == 0) { HTR_LAUNCH_POINT((shock_sensor_kernel<MX, 0>), f, g) }
+   if (dir == 1) { HTR_LAUNCH_POINT((shock_sensor_kernel<MX, 1>), f, g) }
+   if (dir == 2) { HTR_LAUNCH_POINT((shock_sensor_kernel<MX, 2>), f, g) }
+   return set_error(HTR_E_ARG, "shock sensor: bad direction %d", dir);
+}
+
+int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, double dt, hipStream_t s)
+{
+   static const double C[3][3] = { { 1.0, 0.0, 1.0 }, { 3.0/4.0, 1.0/4.0, 1.0/4.0 }, { 1.0/3.0, 2.0/3.0, 2.0/3.0 } };
+   if (stage < 0 || stage > 2) return set_error(HTR_E_ARG, "rk: bad stage %d", stage);
+   HTR_LAUNCH_POINT(rk_update_kernel, f, g, nEq, C[stage][0], C[stage][1], C[stage][2], dt)
+}
+
+int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
+{ HTR_LAUNCH_POINT(rk_init_kernel, f, g, nEq) }
+
+int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
+{ HTR_LAUNCH_POINT(zero_rhs_kernel, f, g, nEq) }
+
+template<class MX>
+int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, double *result, hipStream_t s)
+{
+   const int64_t n = exec_cells(g);
+   Workspace &w = workspace();
+   const int64_t blocks = std::min<int64_t>(std::max<int64_t>(grid_for(n, 256), 1), 4096);
+   hipLaunchKernelGGL(spectral_radius_kernel<MX>, dim3((unsigned)blocks), dim3(256), 0, s, f, mix, g, n, w.partial);
+   int rc = check_launch("spectral_radius_kernel");
+   if (rc) return rc;
+   hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(256), 0, s, w.partial, (int)blocks, w.partial + Workspace::kPartial - 1);
+   rc = check_launch("reduce_max_kernel");
+   if (rc) return rc;
+   if (hipMemcpyAsync(w.result_host, w.partial + Workspace::kPartial - 1, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess)
+      return set_error(HTR_E_DEVICE, "spectral radius: copy back failed");
+   *result = *w.result_host;
+   return HTR_OK;
+}
+
+#define HTR_INST_POINT(MX)                                                                          \
+   template int properties_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);  \
+   template int conserved_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);   \
+   template int primitive_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);   \
+   template int chem_source_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t); \
+   template int shock_sensor_launch<MX>(int, const htr_fields &, const Geo &, hipStream_t);         \
+   template int spectral_radius_launch<MX>(const htr_fields &, const DMix *, const Geo &, double *, hipStream_t);
+HTR_INST_POINT(MixCP)
+HTR_INST_POINT(MixAir)
+HTR_INST_POINT(Mix30)
+
+} // namespace htr

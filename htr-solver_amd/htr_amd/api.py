@@ -1,0 +1,266 @@
+"""Task-level API mirroring the reference's Legion tasks of the hot path.
+
+Each function takes a FieldSet (device tensors of the Fluid_columns members,
+src/prometeo_types.h:48-114), a Mixture and the task's scalar arguments, and
+runs the corresponding C-ABI entry point on the current HIP stream.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import Diag, Fields, Grid, View, check, lib, HTR_CHEM_FP64
+from . import HTR_EULER_HYBRID, HTR_EULER_TENO, HTR_EULER_TENOA, HTR_EULER_TENOLAD, HTR_EULER_KG
+
+AIR_SPECIES = ("N2", "O2", "NO", "N", "O")
+
+__all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
+           "update_properties_from_primitive", "update_conserved_from_primitive",
+           "update_primitive_from_conserved", "update_ducros_sensor", "update_shock_sensor",
+           "update_using_euler_flux", "update_using_hybrid_euler_flux", "update_using_teno_euler_flux",
+           "update_using_skew_symmetric_euler_flux", "update_using_diffusion_flux",
+           "add_chemistry_sources", "update_chemistry", "calculate_max_spectral_radius",
+           "rk_update", "rk_init", "full_rhs", "stream_ptr"]
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class Mixture:
+    """A device mixture image (LoadMixtureTask): ConstPropMix, AirMix or a
+    generic multicomponent mechanism."""
+
+    def __init__(self, handle, nspec):
+        self.h = handle
+        self.nSpec = nspec
+
+    @property
+    def nEq(self):
+        return self.nSpec + 4
+
+    @classmethod
+    def constprop(cls, R, gamma, prandtl, visc_model=0, p0=0.0, p1=0.0, p2=0.0):
+        h = C.c_void_p()
+        check(lib().htr_mix_create_constprop(C.byref(h), R, gamma, prandtl, visc_model, p0, p1, p2))
+        return cls(h, 1)
+
+    @classmethod
+    def airmix(cls, PRef, TRef, LRef, xref):
+        h = C.c_void_p()
+        idx = (C.c_int * len(xref))(*[AIR_SPECIES.index(k) for k in xref])
+        val = (C.c_double * len(xref))(*[xref[k] for k in xref])
+        check(lib().htr_mix_create_airmix(C.byref(h), PRef, TRef, LRef, len(xref), idx, val))
+        return cls(h, 5)
+
+    @classmethod
+    def from_mechanism(cls, mech):
+        ns = len(mech["W"])
+        h = C.c_void_p()
+        a = lambda k, t: np.ascontiguousarray(mech[k], dtype=t)
+        W, cpT, nPoly = a("W", np.float64), a("cpT", np.float32), a("nPoly", np.int32)
+        cpH, cpM, cpL, geom, dc = a("cpH", np.float64), a("cpM", np.float64), a("cpL", np.float64), \
+            a("geom", np.int32), a("dc", np.float64)
+        dp, fp, ip = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
+        check(lib().htr_mix_create_species(C.byref(h), ns, W.ctypes.data_as(dp), cpT.ctypes.data_as(fp),
+                                           nPoly.ctypes.data_as(ip), cpH.ctypes.data_as(dp),
+                                           cpM.ctypes.data_as(dp), cpL.ctypes.data_as(dp),
+                                           geom.ctypes.data_as(ip), dc.ctypes.data_as(dp)))
+        m = cls(h, ns)
+        for r in mech["reactions"]:
+            ei = np.array([e[0] for e in r["educts"]], np.int32)
+            en = np.array([e[1] for e in r["educts"]], np.float32)
+            pi = np.array([e[0] for e in r["products"]], np.int32)
+            pn = np.array([e[1] for e in r["products"]], np.float32)
+            ti = np.array([e[0] for e in r.get("thirdb", [])], np.int32)
+            te = np.array([e[1] for e in r.get("thirdb", [])], np.float32)
+            check(lib().htr_mix_add_reaction(h, r["kind"], r["A"], r["n"], r["EovR"], int(r["has_backward"]),
+                                             len(ei), ei.ctypes.data_as(ip), en.ctypes.data_as(fp),
+                                             len(pi), pi.ctypes.data_as(ip), pn.ctypes.data_as(fp),
+                                             len(ti), ti.ctypes.data_as(ip) if len(ti) else None,
+                                             te.ctypes.data_as(fp) if len(te) else None))
+        xr = np.ascontiguousarray(mech["XiRef"], np.float64)
+        check(lib().htr_mix_finalize(h, mech["PRef"], mech["TRef"], mech["LRef"], xr.ctypes.data_as(dp)))
+        return m
+
+    def refs(self):
+        out = (C.c_double * 9)()
+        check(lib().htr_mix_refs(self.h, out))
+        keys = ("rhoRef", "ieRef", "iuRef", "iCpRef", "imuRef", "ilamRef", "iDiRef", "iwiRef", "MixWRef")
+        return dict(zip(keys, list(out)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().htr_mix_destroy(self.h)
+        except Exception:
+            pass
+
+
+def make_grid(n, lo=(0, 0, 0), hi=None, bounds_lo=(0, 0, 0)):
+    """Grid over a box of n = (nx, ny, nz) cells whose first cell has absolute
+    coordinates bounds_lo; exec rect [lo, hi] in box-relative indices."""
+    g = Grid()
+    hi = hi if hi is not None else tuple(x - 1 for x in n)
+    for d in range(3):
+        g.bounds.lo[d] = bounds_lo[d]
+        g.bounds.hi[d] = bounds_lo[d] + n[d] - 1
+        g.exec.lo[d] = bounds_lo[d] + lo[d]
+        g.exec.hi[d] = bounds_lo[d] + hi[d]
+    return g
+
+
+SCALARS = ("rho", "mu", "lam", "SoS", "pressure", "temperature", "DucrosSensor")
+VECTORS = {"Di": "ns", "MassFracs": "ns", "MolarFracs": "ns", "velocity": 3,
+           "Conserved": "neq", "Conserved_old": "neq", "Conserved_t": "neq", "Conserved_t_old": "neq"}
+
+
+class FieldSet:
+    """Device fields of one tile in structure-of-arrays layout: a field is a
+    (ncomp, nz, ny, nx) tensor, x fastest, components far apart."""
+
+    def __init__(self, n, nspec, device="cuda", bounds_lo=(0, 0, 0), decisions=False, nsteps=False):
+        nx, ny, nz = n
+        self.n = tuple(n)
+        self.ns = nspec
+        self.bounds_lo = tuple(bounds_lo)
+        kw = dict(device=device)
+        self.t = {}
+        for k in SCALARS:
+            self.t[k] = torch.zeros((nz, ny, nx), dtype=torch.float64, **kw)
+        for k, c in VECTORS.items():
+            nc = {"ns": nspec, "neq": nspec + 4}.get(c, c)
+            self.t[k] = torch.zeros((nc, nz, ny, nx), dtype=torch.float64, **kw)
+        for d in range(3):
+            self.t["nType%d" % d] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+            for k in ("m_e", "m_d", "m_s"):
+                self.t["%s%d" % (k, d)] = torch.ones((nz, ny, nx), dtype=torch.float64, **kw)
+            self.t["shockSensor%d" % d] = torch.ones((nz, ny, nx), dtype=torch.uint8, **kw)
+            if decisions:
+                self.t["decisions%d" % d] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+        if nsteps:
+            self.t["nsteps"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+        self._c = None
+
+    def __getitem__(self, k):
+        return self.t[k]
+
+    def __setitem__(self, k, v):
+        if isinstance(v, np.ndarray):
+            v = torch.from_numpy(np.ascontiguousarray(v))
+        self.t[k].copy_(v if torch.is_tensor(v) else torch.tensor(v, dtype=self.t[k].dtype))
+
+    def view(self, name):
+        t = self.t[name]
+        v = View()
+        if t.dim() == 4:
+            sc, sz, sy, sx = t.stride()
+        else:
+            sz, sy, sx = t.stride()
+            sc = 0
+        lo = self.bounds_lo
+        off = lo[0] * sx + lo[1] * sy + lo[2] * sz
+        v.ptr = t.data_ptr() - off * t.element_size()
+        v.sx, v.sy, v.sz, v.sc = sx, sy, sz, sc
+        return v
+
+    def c(self):
+        f = Fields()
+        for d in range(3):
+            f.nType[d] = self.view("nType%d" % d)
+            f.m_e[d] = self.view("m_e%d" % d)
+            f.m_d[d] = self.view("m_d%d" % d)
+            f.m_s[d] = self.view("m_s%d" % d)
+            f.shockSensor[d] = self.view("shockSensor%d" % d)
+        for k in SCALARS + tuple(VECTORS):
+            setattr(f, k, self.view(k))
+        return f
+
+    def diag(self):
+        dg = Diag()
+        for d in range(3):
+            if "decisions%d" % d in self.t:
+                dg.decisions[d] = self.view("decisions%d" % d)
+        if "nsteps" in self.t:
+            dg.nsteps = self.view("nsteps")
+        return dg
+
+    def grid(self, lo=(0, 0, 0), hi=None):
+        return make_grid(self.n, lo, hi, self.bounds_lo)
+
+
+def _g(fs, grid):
+    return C.byref(grid if grid is not None else fs.grid())
+
+
+# ---- tasks --------------------------------------------------------------------
+def update_properties_from_primitive(fs, mix, grid=None, stream=None):
+    check(lib().htr_update_properties(_g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def update_conserved_from_primitive(fs, mix, grid=None, stream=None):
+    check(lib().htr_update_conserved(_g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def update_primitive_from_conserved(fs, mix, grid=None, stream=None):
+    check(lib().htr_update_primitive(_g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def update_ducros_sensor(fs, vorticity_scale, grid=None, stream=None):
+    check(lib().htr_ducros_sensor(_g(fs, grid), C.byref(fs.c()), vorticity_scale, stream_ptr(stream)))
+
+
+def update_shock_sensor(fs, direction, mix, grid=None, stream=None):
+    check(lib().htr_shock_sensor(direction, _g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def update_using_euler_flux(fs, direction, scheme, RK_coeffs, deltaTime, mix, grid=None, stream=None, diag=False):
+    dg = C.byref(fs.diag()) if diag else None
+    check(lib().htr_euler_flux(direction, scheme, _g(fs, grid), C.byref(fs.c()), RK_coeffs[0], RK_coeffs[1],
+                               deltaTime, mix.h, dg, stream_ptr(stream)))
+
+
+def update_using_hybrid_euler_flux(fs, direction, RK_coeffs, deltaTime, mix, **kw):
+    update_using_euler_flux(fs, direction, HTR_EULER_HYBRID, RK_coeffs, deltaTime, mix, **kw)
+
+
+def update_using_teno_euler_flux(fs, direction, op, RK_coeffs, deltaTime, mix, **kw):
+    scheme = {"TENO": HTR_EULER_TENO, "TENOA": HTR_EULER_TENOA, "TENOLAD": HTR_EULER_TENOLAD}[op]
+    update_using_euler_flux(fs, direction, scheme, RK_coeffs, deltaTime, mix, **kw)
+
+
+def update_using_skew_symmetric_euler_flux(fs, direction, mix, **kw):
+    update_using_euler_flux(fs, direction, HTR_EULER_KG, (0.0, 0.0), 0.0, mix, **kw)
+
+
+def update_using_diffusion_flux(fs, direction, mix, grid=None, stream=None):
+    check(lib().htr_diffusion_flux(direction, _g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def add_chemistry_sources(fs, mix, grid=None, stream=None):
+    check(lib().htr_chem_source(_g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def update_chemistry(fs, deltaTime, mix, precision=HTR_CHEM_FP64, grid=None, stream=None, diag=False):
+    dg = C.byref(fs.diag()) if diag else None
+    check(lib().htr_chem_update(_g(fs, grid), C.byref(fs.c()), deltaTime, precision, mix.h, dg, stream_ptr(stream)))
+
+
+def calculate_max_spectral_radius(fs, mix, grid=None, stream=None):
+    out = C.c_double(0.0)
+    check(lib().htr_max_spectral_radius(_g(fs, grid), C.byref(fs.c()), mix.h, C.byref(out), stream_ptr(stream)))
+    return out.value
+
+
+def rk_update(fs, nEq, stage, dt, grid=None, stream=None):
+    check(lib().htr_rk_update(_g(fs, grid), C.byref(fs.c()), nEq, stage, dt, stream_ptr(stream)))
+
+
+def rk_init(fs, nEq, grid=None, stream=None):
+    check(lib().htr_rk_init(_g(fs, grid), C.byref(fs.c()), nEq, stream_ptr(stream)))
+
+
+def full_rhs(fs, scheme, RK_coeffs, deltaTime, chem_source, mix, grid=None, stream=None):
+    check(lib().htr_full_rhs(_g(fs, grid), C.byref(fs.c()), scheme, RK_coeffs[0], RK_coeffs[1], deltaTime,
+                             int(chem_source), mix.h, stream_ptr(stream)))

@@ -1,0 +1,190 @@
+/*
+ * htr_amd.h -- C ABI of the MI355X (gfx950) implementation of the HTR
+ * per-cell right-hand-side hot path.
+ *
+ * Each entry point replaces the GPU variant (gpu_base_impl) of one reference
+ * Legion task; the citation next to each declaration names it.  A
+ * gpu_base_impl body that calls these functions extracts, from its
+ * PhysicalRegions, one htr_view per field (base address of point (0,0,0) plus
+ * element strides, exactly what Realm::AffineAccessor holds) and forwards the
+ * task's scalar arguments unchanged.
+ *
+ * Conventions
+ *   - coordinates are absolute Legion coordinates; htr_grid.bounds is the
+ *     task's Fluid_bounds argument (periodic wrap is done over it, as
+ *     warpPeriodic does), htr_grid.exec is the index space the task runs
+ *     over (the domain of its last region argument);
+ *   - all pointers are device pointers; no function allocates per call or
+ *     synchronises the stream, except htr_max_spectral_radius, which returns
+ *     the maximum to the host (the reference returns a future);
+ *   - every function returns 0 on success or a negative HTR_E_* code;
+ *     htr_last_error() describes the last failure of the calling thread;
+ *   - mixtures are uploaded once (htr_mix_*), as LoadMixtureTask does.
+ */
+#ifndef HTR_AMD_H
+#define HTR_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HTR_OK                  0
+#define HTR_E_ARG              -1   /* null pointer, bad direction, bad rect */
+#define HTR_E_UNSUPPORTED      -2   /* no kernel instance for this species count */
+#define HTR_E_DEVICE           -3   /* a HIP call failed */
+#define HTR_E_FIELD            -4   /* a required field view is missing */
+#define HTR_E_NUMERIC          -5   /* a per-cell solver did not converge */
+
+/* Euler schemes (src/config_schema.lua:389-397) */
+#define HTR_EULER_HYBRID   0
+#define HTR_EULER_TENO     1
+#define HTR_EULER_TENOA    2
+#define HTR_EULER_TENOLAD  3
+#define HTR_EULER_KG       4   /* skew-symmetric (Kennedy-Gruber) */
+
+/* implicit-chemistry linear-solve precision */
+#define HTR_CHEM_FP64      0   /* FP64 LU, as the reference */
+#define HTR_CHEM_FP32_IR   1   /* FP32 LU + FP64 residual iterative refinement */
+
+typedef struct {
+   void   *ptr;                /* address of element (0,0,0), component 0 */
+   int64_t sx, sy, sz;         /* strides in elements */
+   int64_t sc;                 /* component stride in elements (vector fields) */
+} htr_view;
+
+typedef struct {
+   int64_t lo[3];
+   int64_t hi[3];              /* inclusive */
+} htr_rect;
+
+typedef struct {
+   htr_rect bounds;            /* Fluid_bounds */
+   htr_rect exec;              /* execution index space */
+} htr_grid;
+
+/* One view per Fluid_columns member used by the hot path
+ * (src/prometeo_types.h:48-114).  Element types: nType int32, shockSensor
+ * uint8 (bool), everything else float64.  Views a task does not read may be
+ * left zeroed. */
+typedef struct {
+   htr_view nType[3];          /* nType_x, nType_y, nType_z */
+   htr_view m_e[3];            /* dcsi_e, deta_e, dzet_e */
+   htr_view m_d[3];            /* dcsi_d, deta_d, dzet_d */
+   htr_view m_s[3];            /* dcsi_s, deta_s, dzet_s */
+   htr_view rho, mu, lam, Di, SoS;
+   htr_view pressure, temperature, MassFracs, MolarFracs, velocity;
+   htr_view Conserved, Conserved_old, Conserved_t, Conserved_t_old;
+   htr_view shockSensor[3];    /* shockSensorX/Y/Z */
+   htr_view DucrosSensor;
+} htr_fields;
+
+/* Optional diagnostics written by instrumented calls (any view may be null):
+ *   decisions[d]: uint32 per cell, the decision word of the Euler face on the
+ *                 cell's + side (bit0 KG used, bit1 limiter fired, bits 2..31
+ *                 fold of the TENO stencil-selection bits);
+ *   nsteps:       int32 per cell, Rosenbrock steps (negative: cap reached). */
+typedef struct {
+   htr_view decisions[3];
+   htr_view nsteps;
+} htr_diag;
+
+typedef struct htr_mix htr_mix;
+
+/* ---- library / device ----------------------------------------------------- */
+int         htr_init(int device);
+const char *htr_last_error(void);
+const char *htr_version(void);
+
+/* ---- mixtures (src/prometeo_mixture.cu:33-47 LoadMixtureTask) -------------- */
+/* ConstPropMix (src/Mixtures/ConstPropMix.inl:30-50); visc_model 0 constant
+ * (p0 = viscosity), 1 power law (p0 = TempRef, p1 = ViscRef), 2 Sutherland
+ * (p0 = SRef, p1 = TempRef, p2 = ViscRef) */
+int htr_mix_create_constprop(htr_mix **out, double R, double gamma, double prandtl,
+                             int visc_model, double p0, double p1, double p2);
+/* AirMix (src/Mixtures/AirMix.hpp:318-333): species N2, O2, NO, N, O; XiRef
+ * given as nref (species index, molar fraction) pairs */
+int htr_mix_create_airmix(htr_mix **out, double PRef, double TRef, double LRef,
+                          int nref, const int *ref_species, const double *ref_X);
+/* Generic multicomponent mixture of the MultiComponent.hpp family from flat
+ * arrays (W[ns]; cpT[4*ns] = TSwitch1,TSwitch2,TMin,TMax; nPoly[ns] (2|3);
+ * cpH,cpM,cpL[9*ns]; geom[ns]; dc[5*ns] = sigma,kbOveps,mu,alpha,Z298) and
+ * reactions added with htr_mix_add_reaction, then htr_mix_finalize. */
+int htr_mix_create_species(htr_mix **out, int ns, const double *W, const float *cpT,
+                           const int *nPoly, const double *cpH, const double *cpM,
+                           const double *cpL, const int *geom, const double *dc);
+/* kind 0 standard, 1 third body (src/Mixtures/Reaction.hpp) */
+int htr_mix_add_reaction(htr_mix *m, int kind, double A, float n, float EovR,
+                         int has_backward, int Ne, const int *ed_ind, const float *ed_nu,
+                         int Np, const int *pd_ind, const float *pd_nu,
+                         int Ntb, const int *tb_ind, const float *tb_eff);
+/* StoreReferenceQuantities (src/Mixtures/MultiComponent.inl:438-488) + upload */
+int htr_mix_finalize(htr_mix *m, double PRef, double TRef, double LRef, const double *XiRef);
+int htr_mix_nspec(const htr_mix *m);
+/* reference scales: rhoRef, ieRef, iuRef, iCpRef, imuRef, ilamRef, iDiRef, iwiRef, MixWRef */
+int htr_mix_refs(const htr_mix *m, double *out9);
+void htr_mix_destroy(htr_mix *m);
+
+/* ---- variables (src/prometeo_variables.hpp) ------------------------------- */
+/* UpdatePropertiesFromPrimitiveTask::gpu_base_impl (src/prometeo_variables.cu:41-130) */
+int htr_update_properties(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+/* UpdateConservedFromPrimitiveTask::gpu_base_impl (src/prometeo_variables.cu:132-200) */
+int htr_update_conserved(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+/* UpdatePrimitiveFromConservedTask::gpu_base_impl (src/prometeo_variables.cu:202-262) */
+int htr_update_primitive(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+
+/* ---- sensors (src/prometeo_sensor.hpp) ------------------------------------ */
+/* UpdateDucrosSensorTask::gpu_base_impl (src/prometeo_sensor.cu:39-119) */
+int htr_ducros_sensor(const htr_grid *g, const htr_fields *f, double vorticityScale, void *stream);
+/* UpdateShockSensorTask<dir>::gpu_base_impl (src/prometeo_sensor.cu:121-220) */
+int htr_shock_sensor(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+
+/* ---- Euler fluxes (src/prometeo_rhs.hpp:171-366) -------------------------- */
+/* scheme HTR_EULER_HYBRID: UpdateUsingHybridEulerFluxTask<dir>::gpu_base_impl
+ *   (src/prometeo_rhs.cu:98-160); TENO/TENOA/TENOLAD:
+ *   UpdateUsingTENOEulerFluxTask<dir,Op> (src/prometeo_rhs.cu:237-298);
+ *   KG: UpdateUsingSkewSymmetricEulerFluxTask<dir> (src/prometeo_rhs.cu:399-440).
+ * RK0, RK1, deltaTime are the task's RK_coeffs[2] and deltaTime arguments. */
+int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
+                   double RK0, double RK1, double deltaTime, const htr_mix *m,
+                   const htr_diag *diag, void *stream);
+
+/* ---- diffusion (src/prometeo_rhs.hpp:372-473) ----------------------------- */
+/* UpdateUsingDiffusionFluxTask<dir>::gpu_base_impl (src/prometeo_rhs.cu:530-604) */
+int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+
+/* ---- chemistry (src/prometeo_chem.hpp) ------------------------------------ */
+/* AddChemistrySourcesTask::gpu_base_impl (src/prometeo_chem.cu:131-162) */
+int htr_chem_source(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+/* UpdateChemistryTask::gpu_base_impl (src/prometeo_chem.cu:65-98);
+ * precision HTR_CHEM_FP64 or HTR_CHEM_FP32_IR.  Returns HTR_E_NUMERIC (after
+ * finishing every cell) when any cell hit the reference's step caps. */
+int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int precision,
+                    const htr_mix *m, const htr_diag *diag, void *stream);
+
+/* ---- CFL (src/prometeo_cfl.hpp:55-190) ------------------------------------ */
+/* CalculateMaxSpectralRadiusTask::gpu_base_impl (src/prometeo_cfl.cu:97-152);
+ * blocks on the stream and writes the maximum to *result */
+int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mix *m,
+                            double *result, void *stream);
+
+/* ---- time integration (src/prometeo_rk.rg:48-112) ------------------------- */
+/* SSP-RK3 stage update: Conserved = c1*Conserved_old + c2*Conserved + c3*dt*Conserved_t
+ * with (c1,c2,c3) = RK_C[stage] of src/prometeo_const.rg:54-58, stage 0..2 */
+int htr_rk_update(const htr_grid *g, const htr_fields *f, int nEq, int stage, double dt, void *stream);
+/* Conserved_old = Conserved (InitializeTemporaries) */
+int htr_rk_init(const htr_grid *g, const htr_fields *f, int nEq, void *stream);
+
+/* ---- one full right-hand side of an RK stage ------------------------------ */
+/* The sequence of src/prometeo.rg:503-578 on one tile: Conserved_t = 0,
+ * chemistry source (if chem_source), Euler Z, Y, X, diffusion Z, Y, X;
+ * same accumulation order as the reference tasks. */
+int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
+                 double deltaTime, int chem_source, const htr_mix *m, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

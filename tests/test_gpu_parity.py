@@ -1,0 +1,298 @@
+"""GPU parity: every hot-path task through the C ABI against the oracle on
+identical seeded inputs.
+
+Bar (north star): FP64 results within 1e-12 of the oracle, normalised per
+conserved component (max |gpu - oracle| / max |oracle|), over cells whose
+rounding-sensitive decisions agree; decision flips (TENO stencil selection,
+hybrid branch, limiter) are counted and printed, and the flipped cells are
+excluded from the 1e-12 claim but must stay below 1e-6.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from hotpath import TOL, air_state, complete_state, dt_cfl, flux_scale, gpu_mix, norm_err, oracle_fields, \
+    tgv_state, to_gpu, wall_state
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import htr_amd
+    htr_amd.init(0)
+    yield
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+SCHEMES = [(O.SCHEME_HYBRID, "hybrid"), (O.SCHEME_TENO, "TENO"), (O.SCHEME_TENOA, "TENOA"),
+           (O.SCHEME_TENOLAD, "TENOLAD"), (O.SCHEME_KG, "KG")]
+
+
+def euler_case(state, meta, n, om, d, scheme, RK=(1.0, 0.0), dt=1e-3, exec_lo=(0, 0, 0), exec_hi=None):
+    import htr_amd as H
+    nspec = om.nSpec
+    g, f = oracle_fields(state, nspec, n)
+    g = g.with_rect(exec_lo, exec_hi if exec_hi else tuple(x - 1 for x in n))
+    f["Conserved_t"] = 0.0
+    dec_o = np.zeros(f["rho"].shape, np.uint32)
+    O.euler_flux(g, f, d, scheme, RK[0], RK[1], dt, om, dec_o)
+    fs = to_gpu(state, nspec, n, decisions=True)
+    fs["Conserved_t"] = 0.0
+    gm = gpu_mix(meta)
+    H.update_using_euler_flux(fs, d, scheme, RK, dt, gm, grid=fs.grid(exec_lo, exec_hi), diag=True)
+    ct = host(fs["Conserved_t"])
+    dec_g = host(fs["decisions%d" % d]).astype(np.uint32)
+    return f["Conserved_t"], ct, dec_o, dec_g
+
+
+def check_euler(ref, gpu, dec_o, dec_g, label, d, floor=None):
+    # a cell is affected by a flipped decision if either of its two faces flipped
+    flip_face = dec_o != dec_g
+    flip_cell = flip_face | np.roll(flip_face, 1, axis=2 - d)
+    nflip = int(flip_face.sum())
+    e = norm_err(gpu, ref, mask=~flip_cell, floor=floor)
+    e_all = norm_err(gpu, ref, floor=floor)
+    if nflip:
+        idx = np.argwhere(flip_face)[:4]
+        for i in idx:
+            a, b = int(dec_o[tuple(i)]), int(dec_g[tuple(i)])
+            say("   flip at %s: oracle %08x gpu %08x xor %08x" % (tuple(i), a, b, a ^ b))
+    teno_faces = int(np.sum((dec_o & 1) == 0))
+    say("%s dir %d: max norm err %.3e (all cells %.3e), TENO faces %d, decision flips %d"
+        % (label, d, max(e), max(e_all), teno_faces, nflip))
+    assert max(e) <= TOL, (label, d, e)
+    assert max(e_all) <= 1e-6, (label, d, e_all)
+    return nflip
+
+
+@pytest.fixture(scope="module")
+def tgv16():
+    st, meta = tgv_state(16)
+    n = (16, 16, 16)
+    st, om = complete_state(st, meta, n)
+    return st, meta, n, om
+
+
+@pytest.fixture(scope="module")
+def air16():
+    st, meta = air_state(16, seed=0)
+    n = (16, 16, 16)
+    st, om = complete_state(st, meta, n)
+    return st, meta, n, om
+
+
+def test_properties_conserved_primitive_air(air16):
+    import htr_amd as H
+    st, meta, n, om = air16
+    say("case air16 properties/conserved/primitive")
+    fs = to_gpu({k: st[k] for k in ("pressure", "temperature", "MolarFracs", "velocity")}, 5, n)
+    gm = gpu_mix(meta)
+    H.update_properties_from_primitive(fs, gm)
+    for k in ("rho", "mu", "lam", "Di", "SoS", "MassFracs"):
+        e = norm_err(host(fs[k]), st[k])
+        say("  %s: %.3e" % (k, max(e)))
+        assert max(e) <= TOL, k
+    # conserved from the oracle's properties
+    fs2 = to_gpu(st, 5, n)
+    fs2["Conserved"] = 0.0
+    H.update_conserved_from_primitive(fs2, gm)
+    e = norm_err(host(fs2["Conserved"]), st["Conserved"])
+    say("  Conserved: %.3e" % max(e))
+    assert max(e) <= TOL
+    # primitive recompute: Newton from a perturbed temperature guess
+    g, f = oracle_fields(st, 5, n)
+    f["temperature"] = st["temperature"] * 0.9
+    O.update_primitive(g, f, om)
+    fs3 = to_gpu(st, 5, n)
+    fs3["temperature"] = torch.from_numpy(st["temperature"] * 0.9)
+    H.update_primitive_from_conserved(fs3, gm)
+    for k in ("temperature", "pressure", "MolarFracs", "velocity"):
+        e = norm_err(host(fs3[k]), f[k])
+        say("  %s: %.3e" % (k, max(e)))
+        assert max(e) <= 1e-9, k   # Newton stop test is |dT| < 1e-8: an extra iteration moves T by < 1e-8
+
+
+def test_sensors_air(air16):
+    import htr_amd as H
+    st, meta, n, om = air16
+    say("case air16 sensors")
+    fs = to_gpu(st, 5, n)
+    gm = gpu_mix(meta)
+    fs["DucrosSensor"] = 0.0
+    H.update_ducros_sensor(fs, 1.0)
+    e = norm_err(host(fs["DucrosSensor"]), st["DucrosSensor"])
+    say("  Ducros: %.3e" % max(e))
+    assert max(e) <= TOL
+    for d in range(3):
+        fs["shockSensor%d" % d] = 0
+        H.update_shock_sensor(fs, d, gm)
+        got = host(fs["shockSensor%d" % d])
+        flips = int(np.sum(got != st["shockSensor%d" % d]))
+        say("  shock sensor dir %d: smooth cells %d/%d, flips %d" % (d, int(got.sum()), got.size, flips))
+        assert flips <= max(1, got.size // 1000)
+
+
+@pytest.mark.parametrize("scheme,label", SCHEMES)
+def test_euler_tgv(tgv16, scheme, label):
+    st, meta, n, om = tgv16
+    say("case tgv16 euler", label)
+    for d in range(3):
+        ref, gpu, do, dg = euler_case(st, meta, n, om, d, scheme)
+        check_euler(ref, gpu, do, dg, "tgv16 " + label, d, floor=flux_scale(st, d))
+
+
+@pytest.mark.parametrize("scheme,label", SCHEMES)
+def test_euler_air(air16, scheme, label):
+    st, meta, n, om = air16
+    say("case air16 euler", label)
+    dt = dt_cfl(st, meta, n)
+    for d in range(3):
+        ref, gpu, do, dg = euler_case(st, meta, n, om, d, scheme, RK=(0.75, 0.25), dt=dt * 0.25)
+        check_euler(ref, gpu, do, dg, "air16 " + label, d)
+
+
+def test_euler_air_forced_teno_limiter(air16):
+    """All faces TENO (sensor false everywhere) with a large stage dt so that
+    the positivity limiter fires on some faces."""
+    st, meta, n, om = air16
+    say("case air16 forced TENO + limiter")
+    st2 = dict(st)
+    for d in range(3):
+        st2["shockSensor%d" % d] = np.zeros_like(st["shockSensor%d" % d])
+    dt = dt_cfl(st, meta, n) * 40.0
+    for d in range(3):
+        ref, gpu, do, dg = euler_case(st2, meta, n, om, d, O.SCHEME_HYBRID, RK=(1.0, 0.0), dt=dt)
+        lim = int(np.sum((do & 2) != 0))
+        say("  limiter faces dir %d: %d" % (d, lim))
+        check_euler(ref, gpu, do, dg, "air16 forced TENO", d)
+
+
+def test_diffusion(tgv16, air16):
+    import htr_amd as H
+    for name, case in (("tgv16", tgv16), ("air16", air16)):
+        st, meta, n, om = case
+        say("case %s diffusion" % name)
+        for d in range(3):
+            g, f = oracle_fields(st, om.nSpec, n)
+            f["Conserved_t"] = 0.0
+            O.diffusion_flux(g, f, d, om)
+            fs = to_gpu(st, om.nSpec, n)
+            fs["Conserved_t"] = 0.0
+            H.update_using_diffusion_flux(fs, d, gpu_mix(meta))
+            e = norm_err(host(fs["Conserved_t"]), f["Conserved_t"])
+            say("  dir %d: %.3e" % (d, max(e)))
+            assert max(e) <= TOL
+
+
+def test_chem_source_and_cfl(air16):
+    import htr_amd as H
+    st, meta, n, om = air16
+    say("case air16 chemistry source + spectral radius")
+    g, f = oracle_fields(st, 5, n)
+    f["Conserved_t"] = 0.0
+    O.chem_source(g, f, om)
+    fs = to_gpu(st, 5, n)
+    fs["Conserved_t"] = 0.0
+    gm = gpu_mix(meta)
+    H.add_chemistry_sources(fs, gm)
+    e = norm_err(host(fs["Conserved_t"])[:5], f["Conserved_t"][:5])
+    say("  source: %.3e" % max(e))
+    assert max(e) <= TOL
+    r_o = O.max_spectral_radius(g, f, om)
+    r_g = H.calculate_max_spectral_radius(fs, gm)
+    say("  radius gpu %.17g oracle %.17g" % (r_g, r_o))
+    assert abs(r_g - r_o) <= 1e-13 * abs(r_o)
+
+
+def test_full_rhs_air(air16):
+    import htr_amd as H
+    st, meta, n, om = air16
+    say("case air16 full RHS (source + Euler ZYX + diffusion ZYX)")
+    dt = dt_cfl(st, meta, n)
+    g, f = oracle_fields(st, 5, n)
+    O.full_rhs(g, f, O.SCHEME_HYBRID, 1.0, 0.0, dt, 1, om)
+    fs = to_gpu(st, 5, n)
+    H.full_rhs(fs, 0, (1.0, 0.0), dt, True, gpu_mix(meta))
+    e = norm_err(host(fs["Conserved_t"]), f["Conserved_t"])
+    say("  full RHS: %s" % ["%.2e" % x for x in e])
+    assert max(e) <= TOL
+
+
+def test_tgv_rk3_steps(tgv16):
+    """Ten SSP-RK3 steps of the TGV, hybrid scheme, dt 5e-3 (config 1 at 16^3):
+    primitive/property recompute, sensors, RHS and RK update all on the GPU,
+    against the oracle doing the same sequence."""
+    import htr_amd as H
+    st, meta, n, om = tgv16
+    say("case tgv16 10 RK3 steps")
+    dt = meta["dt"]
+    g, f = oracle_fields(st, 1, n)
+    fs = to_gpu(st, 1, n)
+    gm = gpu_mix(meta)
+    RKC = [(1.0, 0.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0)]
+    for step in range(10):
+        O.ducros_sensor(g, f, 1.0)
+        for d in range(3):
+            O.shock_sensor(g, f, d, om)
+        H.update_ducros_sensor(fs, 1.0)
+        for d in range(3):
+            H.update_shock_sensor(fs, d, gm)
+        f["Conserved_old"] = f["Conserved"]
+        H.rk_init(fs, 5)
+        for s in range(3):
+            c = RKC[s]
+            O.full_rhs(g, f, O.SCHEME_HYBRID, c[0], c[1], dt * c[2], 0, om)
+            H.full_rhs(fs, 0, (c[0], c[1]), dt * c[2], False, gm)
+            O.rk_update(g, f, 5, s, dt)
+            H.rk_update(fs, 5, s, dt)
+            O.update_primitive(g, f, om)
+            O.update_properties(g, f, om)
+            H.update_primitive_from_conserved(fs, gm)
+            H.update_properties_from_primitive(fs, gm)
+    e = norm_err(host(fs["Conserved"]), f["Conserved"])
+    say("  after 10 steps: %s" % ["%.2e" % x for x in e])
+    assert max(e) <= 1e-11
+
+
+def test_wall_bounded_node_types():
+    """y bounded by staggered walls on a stretched grid: every staggered node
+    type (L_S, Lp1_S, Lp2_S, Rm3_S, Rm2_S, Rm1_S, R_S) on the Euler and
+    diffusion paths, execution rect = interior cells in y."""
+    import htr_amd as H
+    st, meta, n = wall_state(12, 14, 8)
+    say("case wall 12x16x8 node types")
+    st, om = complete_state(st, meta, n)
+    dt = dt_cfl(st, meta, n) * 0.3
+    hi = (n[0] - 1, n[1] - 2, n[2] - 1)
+    for scheme, label in ((O.SCHEME_HYBRID, "hybrid"), (O.SCHEME_TENO, "TENO"), (O.SCHEME_KG, "KG")):
+        ref, gpu, do, dg = euler_case(st, meta, n, om, 1, scheme, RK=(1.0, 0.0), dt=dt,
+                                      exec_lo=(0, 1, 0), exec_hi=hi)
+        check_euler(ref, gpu, do, dg, "wall " + label, 1)
+    for d in range(3):
+        g, f = oracle_fields(st, 5, n)
+        g = g.with_rect((0, 1, 0), hi)
+        f["Conserved_t"] = 0.0
+        O.diffusion_flux(g, f, d, om)
+        fs = to_gpu(st, 5, n)
+        fs["Conserved_t"] = 0.0
+        H.update_using_diffusion_flux(fs, d, gpu_mix(meta), grid=fs.grid((0, 1, 0), hi))
+        e = norm_err(host(fs["Conserved_t"]), f["Conserved_t"])
+        say("  wall diffusion dir %d: %.3e" % (d, max(e)))
+        assert max(e) <= TOL
