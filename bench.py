@@ -1,15 +1,19 @@
 """Benchmark of the HTR right-hand-side hot path on one MI355X.
 
 Workload (one "step" = one full RHS of an RK stage): 256^3 periodic box of
-5-species air (Dirichlet(8,2,.5,.5,.5) mass fractions per cell, T0 ~ U(2500,
-7000) K with 5 % noise, p = 1e4 Pa, k^-5/3 solenoidal velocity at Mach_rms
-0.4), hybrid TENO/KG Euler fluxes + multicomponent diffusion + chemistry
-source, FP64, in the reference accumulation order.  Inputs are resident in
-HBM before the timed region.
+5-species air (Dirichlet(8,2,.5,.5,.5) mass fractions drawn per seed, T0 ~
+U(2500, 7000) K with 5 % Gaussian noise, p = 1e4 Pa, k^-5/3 solenoidal
+velocity at Mach_rms 0.4), hybrid TENO/KG Euler fluxes + multicomponent
+diffusion + chemistry source, FP64, in the reference accumulation order.
+Inputs are resident in HBM before the timed region.  --composition per-cell
+draws the mass fractions independently in every cell (a TENO-heavy stress
+reading).
 
-Prints one JSON line: metric/value (RHS cell-updates/s), the roofline of the
-dominant kernel (HIP events on the library's stream) and a CPU baseline (the
-oracle restatement on the host cores, bounded sample).
+Prints one JSON line: metric/value (RHS cell-updates/s), per-kernel times
+from HIP events recorded by the library around each launch on its stream,
+the roofline of the dominant kernel, the whole-RHS fraction of the HBM
+roofline at the 495 B/cell contract, and a CPU baseline (the oracle
+restatement on the host cores, bounded sample).
 """
 import argparse
 import json
@@ -23,19 +27,36 @@ for _p in (ROOT, os.path.join(ROOT, "htr-solver_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 
-# Algorithmic bytes per cell (FP64 fields read once, Conserved_t written once):
-#   full RHS (AirMix): 51 doubles read + 3 int32 node types + 3 bool sensors
-#   + 9 doubles written = 495 B/cell
-#   one Euler direction (AirMix): Conserved 9, Conserved_old 9, rho, SoS,
-#   MassFracs 5, velocity 3, pressure, m_e (29 doubles), nType int32, sensor
-#   bool, Conserved_t read-modify-write 9 doubles x 2 = 389 B/cell
-#   one diffusion direction (AirMix): Conserved[0:5], T, X 5, rho, mu, lam,
-#   Di 5, velocity 3, m_s, m_d, m_d1, m_d2 (25 doubles), 3 int32 node types,
-#   Conserved_t RMW 144 B = 356 B/cell
-#   chemistry source: rho, p, T, Y 5 (64 B) + Conserved_t[0:5] RMW (80 B) = 144 B/cell
-#   zero Conserved_t: 72 B/cell
-BYTES_PER_CELL = {"rhs": 495.0, "euler": 389.0, "diffusion": 356.0, "chem_source": 144.0, "zero": 72.0}
+# Compulsory (algorithmic) HBM bytes of one full RHS cell-update (AirMix):
+# 51 doubles read once + 3 int32 node types + 3 bool sensors + Conserved_t
+# (9 doubles) written once = 495 B.
+RHS_BYTES_PER_CELL = 495.0
+
+# Per-kernel algorithmic bytes per unit (AirMix, NS = 5, nEq = 9); the unit
+# count of one launch comes from the workload (cells, faces, TENO faces).
+#   euler_kg        per face: own-cell rho, Y 5, u 3, p, rhoE (88 B) + nType
+#                   + sensor (5 B) read, face flux (72 B) written = 165 B
+#   euler_roe       per TENO face: own-cell Roe inputs (88 B) + SoS (8 B)
+#                   read, face record (22 doubles, 176 B) written = 272 B
+#   euler_char      per (component, TENO face): 1/9 of the record (20 B),
+#                   component + bits written (9 B) = 29 B; FP64-bound, see flops
+#   euler_finish    per TENO face: record 176 B + components 72 B + bits 9 B
+#                   read, face flux 72 B written = 329 B
+#   euler_divergence per cell: face flux 72 B + m_e 8 B read, Conserved_t
+#                   read-modify-write 144 B = 224 B
+#   diffusion       per cell: Conserved[0:5], T, X 5, rho, mu, lam, Di 5,
+#                   velocity 3, 4 metrics (25 doubles) + 3 node types read,
+#                   Conserved_t read-modify-write 144 B = 356 B
+#   chem_source     per cell: rho, p, T, Y 5 (64 B) + Conserved_t[0:5] RMW
+#                   (80 B) = 144 B
+KERNEL_BYTES = {"euler_kg": 165.0, "euler_roe": 272.0, "euler_char": 29.0, "euler_finish": 329.0,
+                "euler_divergence": 224.0, "diffusion": 356.0, "chem_source_kernel": 144.0}
+# FP64 operations per unit of the compute-bound kernels (hand count of the
+# source: split fluxes of six points 528, two TENO reconstructions ~300, the
+# projection constants ~20)
+KERNEL_FLOPS = {"euler_char": 850.0}
 
 
 def parse():
@@ -49,15 +70,16 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=48, help="edge of the CPU-baseline box")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
     return ap.parse_args()
 
 
-def setup_gpu(n, seed):
+def setup_gpu(n, seed, composition="per-seed"):
     import numpy as np
     import torch
     import htr_amd as H
     from htr_amd import cases
-    st, meta = cases.air_box(n, seed)
+    st, meta = cases.air_box(n, seed, composition=composition)
     mix = H.Mixture.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
     fs = H.FieldSet((n, n, n), 5)
     for k, v in st.items():
@@ -75,42 +97,46 @@ def setup_gpu(n, seed):
     return fs, mix, dt, meta
 
 
-def run_rhs(H, fs, mix, scheme, dt, events=None):
-    """One full RHS, launched task by task so each launch can be timed."""
-    import torch
-    lib = H.lib()
+def run_rhs(H, fs, mix, scheme, dt):
+    """One full RHS, launched task by task in the reference order (zero,
+    chemistry source, Euler Z Y X, diffusion Z Y X) on torch's current stream."""
     import ctypes as C
+    lib = H.lib()
     g = C.byref(fs.grid())
     f = C.byref(fs.c())
     s = H.stream_ptr()
-    marks = []
-
-    def mark(name):
-        if events is not None:
-            e = torch.cuda.Event(enable_timing=True)
-            e.record()
-            marks.append((name, e))
-
-    mark("start")
     fs["Conserved_t"].zero_()
-    mark("zero")
     H.check(lib.htr_chem_source(g, f, mix.h, s))
-    mark("chem_source")
     for d in (2, 1, 0):
         H.check(lib.htr_euler_flux(d, scheme, g, f, 1.0, 0.0, dt, mix.h, None, s))
-        mark("euler%d" % d)
     for d in (2, 1, 0):
         H.check(lib.htr_diffusion_flux(d, g, f, mix.h, s))
-        mark("diffusion%d" % d)
-    if events is not None:
-        events.append(marks)
 
 
-def cpu_baseline(n, seed, scheme, budget_s):
+def face_counts(H, fs, mix, scheme, dt, n):
+    """TENO and limiter face counts per direction (one instrumented pass,
+    outside the timed region)."""
+    import torch
+    import numpy as np
+    fs2 = H.FieldSet((n, n, n), 5, decisions=True)
+    for k, v in fs.t.items():
+        if k in fs2.t:
+            fs2.t[k].copy_(v)
+    out = {}
+    for d in range(3):
+        H.update_using_euler_flux(fs2, d, scheme, (1.0, 0.0), dt, mix, diag=True)
+        dec = fs2["decisions%d" % d].cpu().numpy().astype(np.uint32)
+        out[d] = (int(np.sum((dec & 1) == 0)), int(np.sum((dec & 2) != 0)))
+    del fs2
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(n, seed, scheme, budget_s, composition="per-seed"):
     import numpy as np
     from htr_amd import cases
     from oracle import oracle as O
-    st, meta = cases.air_box(n, seed)
+    st, meta = cases.air_box(n, seed, composition=composition)
     om = O.Mix.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
     g = O.Grid(n, n, n)
     f = O.Fields(g, 5)
@@ -133,11 +159,32 @@ def cpu_baseline(n, seed, scheme, budget_s):
     return reps * n ** 3 / el, reps, el
 
 
+def kernel_family(name):
+    for k in KERNEL_BYTES:
+        if name.startswith(k):
+            return k
+    return None
+
+
+def units_of(fam, name, n, counts):
+    """Units one launch of kernel `name` processes."""
+    cells = n ** 3
+    d = {"x": 0, "y": 1, "z": 2}.get(name[-1], 0)
+    teno = counts[d][0] if counts else 0
+    if fam == "euler_kg":
+        return (n + 1) * n * n
+    if fam in ("euler_roe", "euler_finish"):
+        return teno
+    if fam == "euler_char":
+        return 9 * teno
+    return cells
+
+
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's share of the "
+              "domain is the unit of work; see the sharding section of DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import torch
     import htr_amd as H
@@ -146,34 +193,44 @@ def main():
     n = a.n
     cells = n ** 3
     t_setup = time.perf_counter()
-    fs, mix, dt, meta = setup_gpu(n, a.seed)
+    fs, mix, dt, meta = setup_gpu(n, a.seed, a.composition)
     print("setup %.1fs, n=%d, dt=%.3e, T0=%.0f K" % (time.perf_counter() - t_setup, n, dt, meta["T0"]),
           file=sys.stderr, flush=True)
+    counts = face_counts(H, fs, mix, scheme, dt, n)
+    print("TENO faces per direction %s" % counts, file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         run_rhs(H, fs, mix, scheme, dt)
     torch.cuda.synchronize()
-    events = []
+    H.profile(True)
+    H.profile_report()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0.record()
     for _ in range(a.steps):
-        run_rhs(H, fs, mix, scheme, dt, events)
+        run_rhs(H, fs, mix, scheme, dt)
     t1.record()
     torch.cuda.synchronize()
+    rep = H.profile_report()
+    H.profile(False)
     total_ms = t0.elapsed_time(t1)
     ms_step = total_ms / a.steps
-    # per-launch averages
-    per = {}
-    for marks in events:
-        for (n0, e0), (n1, e1) in zip(marks[:-1], marks[1:]):
-            per.setdefault(n1, []).append(e0.elapsed_time(e1))
-    avg = {k: sum(v) / len(v) for k, v in per.items()}
-    dom = max(avg, key=avg.get)
-    kind = "euler" if dom.startswith("euler") else "diffusion" if dom.startswith("diffusion") else dom
-    bpc = BYTES_PER_CELL[kind]
-    achieved = bpc * cells / (avg[dom] * 1e-3) / 1e9
-    kg_frac = float(torch.stack([fs["shockSensor%d" % d].double().mean() for d in range(3)]).mean())
+    per_launch = {k: ms / cnt for k, (cnt, ms) in rep.items()}
+    per_step = {k: ms / a.steps for k, (cnt, ms) in rep.items()}
+    dom = max(per_step, key=per_step.get)
+    fam = kernel_family(dom)
+    units = units_of(fam, dom, n, counts) if fam else cells
+    launch_ms = per_launch[dom]
+    bpu = KERNEL_BYTES.get(fam, 0.0)
+    achieved = bpu * units / (launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_unit": bpu, "units_per_launch": units,
+            "launch_ms": launch_ms}
+    if fam in KERNEL_FLOPS:
+        tf = KERNEL_FLOPS[fam] * units / (launch_ms * 1e-3) / 1e12
+        roof["fp64_tflops"] = tf
+        roof["fp64_frac"] = tf / FP64_PEAK_TFS
+    faces = 3 * (n + 1) * n * n
     result = {
         "metric": "RHS cell-updates/s (256^3 5-species air, full FP64 RHS)",
         "value": cells / (ms_step * 1e-3),
@@ -181,17 +238,20 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "air5_box_%d^3_rhs_%s" % (n, a.scheme), "grid": [n, n, n], "species": 5,
-                   "scheme": a.scheme, "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed},
-        "kernel_ms": {k: round(v, 4) for k, v in sorted(avg.items())},
-        "rhs_hbm_frac_495B": 495.0 * cells / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "smooth_sensor_frac": kg_frac,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell": bpc},
+        "config": {"workload": "air5_box_%d^3_rhs_%s_%s" % (n, a.scheme, a.composition), "grid": [n, n, n],
+                   "species": 5, "scheme": a.scheme, "composition": a.composition,
+                   "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed},
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
+        "teno_face_frac": sum(c[0] for c in counts.values()) / faces,
+        "limiter_faces": {d: c[1] for d, c in counts.items()},
+        "rhs_roofline": {"bytes_per_cell": RHS_BYTES_PER_CELL,
+                         "achieved": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9,
+                         "frac": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "roofline": roof,
     }
     if not a.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        rate, reps, el = cpu_baseline(a.cpu_sample, a.seed, scheme, a.cpu_seconds)
+        rate, reps, el = cpu_baseline(a.cpu_sample, a.seed, scheme, a.cpu_seconds, a.composition)
         result["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                   "sample": "%d^3 box of the same workload, %d full RHS in %.1f s"
                                             % (a.cpu_sample, reps, el)}

@@ -40,24 +40,25 @@ static std::mutex g_mu;
 static Workspace g_ws[16];
 static bool g_ready[16];
 
-int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist)
+static int grow(void **p, int64_t &cap, int64_t need, size_t elem, const char *what)
+{
+   if (need <= cap) return HTR_OK;
+   if (*p) (void)hipFree(*p);
+   *p = nullptr;
+   cap = 0;
+   if (hipMalloc(p, elem*(size_t)need) != hipSuccess)
+      return set_error(HTR_E_DEVICE, "%s allocation (%lld elements) failed", what, (long long)need);
+   cap = need;
+   return HTR_OK;
+}
+
+int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno)
 {
    std::lock_guard<std::mutex> lk(g_mu);
-   if (ndoubles > w.faces_cap) {
-      if (w.faces) (void)hipFree(w.faces);
-      w.faces = nullptr; w.faces_cap = 0;
-      if (hipMalloc(&w.faces, sizeof(double)*ndoubles) != hipSuccess)
-         return set_error(HTR_E_DEVICE, "face buffer allocation (%lld doubles) failed", (long long)ndoubles);
-      w.faces_cap = ndoubles;
-   }
-   if (nlist > w.list_cap) {
-      if (w.list) (void)hipFree(w.list);
-      w.list = nullptr; w.list_cap = 0;
-      if (hipMalloc(&w.list, sizeof(int32_t)*nlist) != hipSuccess)
-         return set_error(HTR_E_DEVICE, "face list allocation failed");
-      w.list_cap = nlist;
-   }
-   return HTR_OK;
+   int rc = grow((void**)&w.faces, w.faces_cap, ndoubles, sizeof(double), "face buffer");
+   if (!rc) rc = grow((void**)&w.list, w.list_cap, nlist, sizeof(int32_t), "face list");
+   if (!rc) rc = grow((void**)&w.teno, w.teno_cap, nteno, sizeof(double), "TENO face buffer");
+   return rc;
 }
 
 Workspace &workspace()
@@ -81,6 +82,34 @@ static int ensure_device()
       return set_error(HTR_E_DEVICE, "workspace allocation failed");
    g_ready[dev & 15] = true;
    return HTR_OK;
+}
+
+// ------------------------------------------------------------- launch timing
+struct ProfRec { const char *name; hipEvent_t a, b; };
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;          // launches recorded since the last report
+static std::vector<hipEvent_t> g_prof_pool;  // reusable events
+
+static hipEvent_t prof_event()
+{
+   if (!g_prof_pool.empty()) { hipEvent_t e = g_prof_pool.back(); g_prof_pool.pop_back(); return e; }
+   hipEvent_t e = nullptr;
+   (void)hipEventCreate(&e);
+   return e;
+}
+
+void prof_begin(hipStream_t s, const char *name)
+{
+   if (!g_prof_on) return;
+   ProfRec r{name, prof_event(), prof_event()};
+   (void)hipEventRecord(r.a, s);
+   g_prof.push_back(r);
+}
+
+void prof_end(hipStream_t s)
+{
+   if (!g_prof_on || g_prof.empty()) return;
+   (void)hipEventRecord(g_prof.back().b, s);
 }
 
 // ------------------------------------------------------------- mixture data
@@ -318,6 +347,40 @@ extern "C" {
 
 const char *htr_last_error(void) { return g_err; }
 const char *htr_version(void) { return "htr-amd 0.1 gfx950"; }
+
+int htr_profile(int enable)
+{
+   g_prof_on = enable != 0;
+   return HTR_OK;
+}
+
+int htr_profile_report(char *buf, size_t len)
+{
+   if (!buf || len == 0) return set_error(HTR_E_ARG, "null buffer");
+   struct Acc { const char *name; int n; double ms; };
+   std::vector<Acc> acc;
+   for (ProfRec &r : g_prof) {
+      if (hipEventSynchronize(r.b) != hipSuccess) return set_error(HTR_E_DEVICE, "event synchronisation failed");
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, r.a, r.b);
+      size_t k = 0;
+      while (k < acc.size() && strcmp(acc[k].name, r.name) != 0) k++;
+      if (k == acc.size()) acc.push_back({r.name, 0, 0.0});
+      acc[k].n++;
+      acc[k].ms += ms;
+      g_prof_pool.push_back(r.a);
+      g_prof_pool.push_back(r.b);
+   }
+   g_prof.clear();
+   size_t pos = 0;
+   buf[0] = 0;
+   for (const Acc &a : acc) {
+      const int w = snprintf(buf + pos, len - pos, "%s %d %.6f\n", a.name, a.n, a.ms);
+      if (w < 0 || (size_t)w >= len - pos) return set_error(HTR_E_ARG, "report buffer too small");
+      pos += (size_t)w;
+   }
+   return HTR_OK;
+}
 
 int htr_init(int device)
 {

@@ -295,9 +295,9 @@ int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, doubl
    if (int rc0 = zero_counters(w.counter, s)) return rc0;
    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, 64), 256*16);
    if (precision == HTR_CHEM_FP64)
-      hipLaunchKernelGGL((chem_update_kernel<MX, HTR_CHEM_FP64>), dim3(blocks), dim3(64), 0, s, f, mix, g, dt, nsteps, w.counter, n);
+      HTR_LAUNCH("chem_update_fp64", (chem_update_kernel<MX, HTR_CHEM_FP64>), dim3(blocks), dim3(64), 0, s, f, mix, g, dt, nsteps, w.counter, n);
    else
-      hipLaunchKernelGGL((chem_update_kernel<MX, HTR_CHEM_FP32_IR>), dim3(blocks), dim3(64), 0, s, f, mix, g, dt, nsteps, w.counter, n);
+      HTR_LAUNCH("chem_update_fp32ir", (chem_update_kernel<MX, HTR_CHEM_FP32_IR>), dim3(blocks), dim3(64), 0, s, f, mix, g, dt, nsteps, w.counter, n);
    int rc = check_launch("chem_update_kernel");
    if (rc) return rc;
    int fails = 0;

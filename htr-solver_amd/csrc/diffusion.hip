@@ -153,7 +153,7 @@ static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipSt
    const int64_t nseg = (g.esz[DIR] + SEG - 1)/SEG;
    const int64_t n = exec_cells(g)/g.esz[DIR]*nseg;
    if (n == 0) return HTR_OK;
-   hipLaunchKernelGGL((diffusion_kernel<MX, DIR, SEG>), dim3(grid_for(n, 256)), dim3(256), 0, s, f, mix, g, n);
+   HTR_LAUNCH(HTR_DIR_NAME("diffusion", DIR), (diffusion_kernel<MX, DIR, SEG>), dim3(grid_for(n, 256)), dim3(256), 0, s, f, mix, g, n);
    return check_launch("diffusion_kernel");
 }
 

@@ -95,7 +95,7 @@ struct Euler {
    // ---- Roe state (rhs.inl:200-279) ----------------------------------------
    struct Roe { double a, e, H, a2, rho, v[3], Y[NS], dpde, dpdrhoi[NS]; };
 
-   __device__ static double mod2(const double *v) { double r = 0.0; r += v[0]*v[0]; r += v[1]*v[1]; r += v[2]*v[2]; return r; }
+   __device__ __forceinline__ static double mod2(const double *v) { double r = 0.0; r += v[0]*v[0]; r += v[1]*v[1]; r += v[2]*v[2]; return r; }
 
    __device__ void roe(Roe &R, const P3 &L, const P3 &Rp) const
    {
@@ -151,9 +151,11 @@ struct Euler {
 
    // Row i of projectToCharacteristicSpace applied to the vector w, with
    // sum1 = sum_j (Coeff - dpdrhoi_j) iaRoe2 w_j and sumQsp = sum_j w_j
-   // precomputed; wm[0..3] are w[NS..NS+3] and ws = w[i-1] for species rows.
-   __device__ static double char_row(int i, const Roe &R, const Proj &P, double sum1, double sumQsp,
-                                     double ws, const double *wm)
+   // precomputed; wm[0..3] are w[NS..NS+3]; for species rows ws = w[i-1] and
+   // Y = Yi[i-1] of the Roe state (passed in, so that no per-lane array is
+   // indexed by the runtime row).
+   __device__ __forceinline__ static double char_row(int i, const Roe &R, const Proj &P, double sum1, double sumQsp,
+                                     double ws, const double *wm, double Y)
    {
       if (i == 0 || i == NEQ-1) {
          const double sg = (i == 0) ? 1.0 : -1.0;
@@ -165,7 +167,6 @@ struct Euler {
          return r*0.5;
       }
       if (i <= NS) {
-         const double Y = R.Y[i-1];
          double r = Y*sum1;
          r += ws;
          r += Y*P.cv[0]*wm[0];
@@ -182,7 +183,7 @@ struct Euler {
    // the scalar q and added to r.  Called for i = 0..NEQ-1 in order, it sums
    // every component in the reference's order: sumQsp collects the species
    // rows and enters the momentum rows where the reference adds it.
-   __device__ static void add_column(double *r, double &sumQsp, int i, double q, const Roe &R,
+   __device__ __forceinline__ static void add_column(double *r, double &sumQsp, int i, double q, const Roe &R,
                                      double ERoe, double dedp)
    {
       if (i == 0) {
@@ -219,110 +220,188 @@ struct Euler {
       }
    }
 
-   // ± split flux of component i at point k (getPlusMinusFlux, rhs.inl:586-620)
-   __device__ void split(double &vp, double &vm, int i, const P3 &P, double un, double pr, double lam,
-                         const Roe &R, const Proj &J, double s1q, double sqq, double s1f, double sqf) const
-   {
-      double qm[4], fm[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) { qm[j] = cons(P, NS+j); fm[j] = qm[j]*un; }
-      fm[iN] += pr;
-      fm[3] += pr*un;
-      const double qs = (i >= 1 && i <= NS) ? cons(P, i-1) : 0.0;
-      const double Q = char_row(i, R, J, s1q, sqq, qs, qm);
-      const double F = char_row(i, R, J, s1f, sqf, qs*un, fm);
-      vp = F + lam*Q;
-      vm = F - lam*Q;
-   }
+   // ---- FluxReconstruction<Op> (rhs.inl:625-759) in three passes ----------
+   // face record: Roe state and the Lax-Friedrichs eigenvalue maxima
+   static constexpr int NREC = 2*NS + 12;
 
-   // FluxReconstruction<Op> (rhs.inl:625-759), one characteristic component
-   // at a time: the TENO result of component i goes straight into the
-   // back-projection, so no per-point split-flux vectors are held.
-   template<int OP>
-   __device__ void teno_flux(double *Flux, const P3 &p, int t, double RK0, double RK1,
-                             double lim_f, uint32_t *dec) const
+   // stencil points pM2..pP3 of the face at p (rhs.inl:640-655)
+   __device__ __forceinline__ void points(P3 *P, const P3 &p, int t) const
    {
-      P3 P[6];
       P[0] = wminus(g, DIR, p, c_st.Cp[t][0]);
       P[1] = wminus(g, DIR, p, c_st.Cp[t][1]);
       P[2] = p;
       P[3] = wplus(g, DIR, p, c_st.Cp[t][2]);
       P[4] = wplus(g, DIR, p, c_st.Cp[t][3]);
       P[5] = wplus(g, DIR, p, c_st.Cp[t][4]);
-      double un[6], pr[6];
-      double Lam1, Lam, LamN;
+   }
+
+   // eigenvalue maxima over the six points (rhs.inl:656-678)
+   __device__ void lam_max(const P3 *P, double &Lam1, double &Lam, double &LamN) const
+   {
 #pragma unroll
       for (int k = 0; k < 6; k++) {
-         un[k] = vel(P[k], iN);
-         pr[k] = ld<double>(f.pressure, P[k]);
+         const double un = vel(P[k], iN);
          const double c = ld<double>(f.SoS, P[k]);
-         const double l1 = fabs(un[k] - c), l = fabs(un[k]), lN = fabs(un[k] + c);
+         const double l1 = fabs(un - c), l = fabs(un), lN = fabs(un + c);
          if (k == 0) { Lam1 = l1; Lam = l; LamN = lN; }
          else { Lam1 = dmax(Lam1, l1); Lam = dmax(Lam, l); LamN = dmax(LamN, lN); }
       }
-      Roe R;
-      roe(R, p, P[3]);
-      Proj J;
+   }
+
+   __device__ static void put_rec(double *__restrict__ rec, int64_t cap, int64_t w, const Roe &R,
+                                  double Lam1, double Lam, double LamN)
+   {
+#pragma unroll
+      for (int s = 0; s < NS; s++) { rec[s*cap + w] = R.Y[s]; rec[(NS+8+s)*cap + w] = R.dpdrhoi[s]; }
+#pragma unroll
+      for (int j = 0; j < 3; j++) rec[(NS+j)*cap + w] = R.v[j];
+      rec[(NS+3)*cap + w] = R.a;
+      rec[(NS+4)*cap + w] = R.H;
+      rec[(NS+5)*cap + w] = R.e;
+      rec[(NS+6)*cap + w] = R.rho;
+      rec[(NS+7)*cap + w] = R.dpde;
+      rec[(2*NS+8)*cap + w] = R.a2;
+      rec[(2*NS+9)*cap + w] = Lam1;
+      rec[(2*NS+10)*cap + w] = Lam;
+      rec[(2*NS+11)*cap + w] = LamN;
+   }
+
+   __device__ __forceinline__ static void get_rec(const double *__restrict__ rec, int64_t cap, int64_t w, Roe &R)
+   {
+#pragma unroll
+      for (int s = 0; s < NS; s++) { R.Y[s] = rec[s*cap + w]; R.dpdrhoi[s] = rec[(NS+8+s)*cap + w]; }
+#pragma unroll
+      for (int j = 0; j < 3; j++) R.v[j] = rec[(NS+j)*cap + w];
+      R.a = rec[(NS+3)*cap + w];
+      R.H = rec[(NS+4)*cap + w];
+      R.e = rec[(NS+5)*cap + w];
+      R.rho = rec[(NS+6)*cap + w];
+      R.dpde = rec[(NS+7)*cap + w];
+      R.a2 = rec[(2*NS+8)*cap + w];
+   }
+
+   // constants of the left-eigenvector rows (rhs.inl:443-455)
+   __device__ __forceinline__ static void proj(Proj &J, const Roe &R)
+   {
       J.iaRoe = 1.0/R.a;
       J.iaRoe2 = 1.0/R.a2;
       J.Coeff = (R.e - 0.5*mod2(R.v))*R.dpde/R.rho;
       J.d = R.dpde/(R.rho*R.a2);
 #pragma unroll
       for (int j = 0; j < 3; j++) J.cv[j] = R.v[j]*J.d;
+   }
+
+   // ± split flux of component i at point P (getPlusMinusFlux,
+   // rhs.inl:586-620): the species sums of the conserved vector and of the
+   // Euler flux, then row i of the left-eigenvector product of each
+   __device__ __forceinline__ void split(double &vp, double &vm, int i, const P3 &P, double lam, const Roe &R,
+                         const Proj &J, double Ysel) const
+   {
+      const double un = vel(P, iN);
+      const double pr = ld<double>(f.pressure, P);
+      double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0;
+#pragma unroll
+      for (int j = 0; j < NS; j++) {
+         const double q = cons(P, j);
+         const double w = (J.Coeff - R.dpdrhoi[j])*J.iaRoe2;
+         s1q += w*q;
+         sqq += q;
+         const double fq = q*un;
+         s1f += w*fq;
+         sqf += fq;
+      }
+      const double qs = (i >= 1 && i <= NS) ? cons(P, i-1) : 0.0;
+      double qm[4], fm[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) { qm[j] = cons(P, NS+j); fm[j] = qm[j]*un; }
+      fm[iN] += pr;
+      fm[3] += pr*un;
+      const double Q = char_row(i, R, J, s1q, sqq, qs, qm, Ysel);
+      const double F = char_row(i, R, J, s1f, sqf, qs*un, fm, Ysel);
+      vp = F + lam*Q;
+      vm = F - lam*Q;
+   }
+
+   // pass A: Roe state and eigenvalue maxima of face p
+   __device__ void face_record(double *__restrict__ rec, int64_t cap, int64_t w, const P3 &p, int t) const
+   {
+      P3 P[6];
+      points(P, p, t);
+      double Lam1, Lam, LamN;
+      lam_max(P, Lam1, Lam, LamN);
+      Roe R;
+      roe(R, p, P[3]);
+      put_rec(rec, cap, w, R, Lam1, Lam, LamN);
+   }
+
+   // pass B: component i of the face, F_i = -(F+ + F-)/2 in characteristic
+   // space, and its stencil-selection bits (plus | minus << 4)
+   template<int OP>
+   __device__ double face_component(const double *__restrict__ rec, int64_t cap, int64_t w, const P3 &p,
+                                    int t, int i, uint32_t *bits) const
+   {
+      P3 P[6];
+      points(P, p, t);
+      Roe R;
+      get_rec(rec, cap, w, R);
+      const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
+      Proj J;
+      proj(J, R);
+      const double Ysel = (i >= 1 && i <= NS) ? rec[(i-1)*cap + w] : 0.0;
+      double vp[6], vm[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) split(vp[k], vm[k], i, P[k], lam, R, J, Ysel);
+      uint32_t bp, bm;
+      const double FPlus  = teno_reconstruct<OP, 0>(vp, t, &bp);
+      const double FMinus = teno_reconstruct<OP, 1>(vm, t, &bm);
+      *bits = bp | (bm << 4);
+      return -0.5*(FPlus + FMinus);
+   }
+
+   // pass C: back-projection (rhs.inl:708) and the positivity limiter
+   // (rhs.inl:710-758); returns the decision word
+   __device__ uint32_t face_finish(double *Flux, const double *__restrict__ rec, int64_t cap, int64_t w,
+                                   const double *__restrict__ Fc, const uint8_t *__restrict__ bits,
+                                   const P3 &p, int t, double RK0, double RK1, double dt) const
+   {
+      Roe R;
+      get_rec(rec, cap, w, R);
       const double dedp = 1.0/R.dpde;
       const double ERoe = R.e + 0.5*mod2(R.v);
-      // per-point species sums of the conserved vector (q) and the Euler flux (f)
-      double s1q[6], sqq[6], s1f[6], sqf[6];
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-         double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-#pragma unroll
-         for (int j = 0; j < NS; j++) {
-            const double q = cons(P[k], j);
-            const double w = (J.Coeff - R.dpdrhoi[j])*J.iaRoe2;
-            a += w*q;
-            b += q;
-            const double fq = q*un[k];
-            c += w*fq;
-            d += fq;
-         }
-         s1q[k] = a; sqq[k] = b; s1f[k] = c; sqf[k] = d;
-      }
-      uint32_t h = 0;
       double sumQsp = 0.0;
-#pragma unroll 1
-      for (int i = 0; i < NEQ; i++) {
-         const double lam = (i == 0) ? Lam1 : (i == NEQ-1) ? LamN : Lam;
-         double vp[6], vm[6];
+      uint32_t h = 0;
 #pragma unroll
-         for (int k = 0; k < 6; k++) split(vp[k], vm[k], i, P[k], un[k], pr[k], lam, R, J, s1q[k], sqq[k], s1f[k], sqf[k]);
-         uint32_t bp, bm;
-         const double FPlus  = teno_reconstruct<OP, 0>(vp, t, &bp);
-         const double FMinus = teno_reconstruct<OP, 1>(vm, t, &bm);
-         h = fold_bits(fold_bits(h, bp), bm);
-         add_column(Flux, sumQsp, i, -0.5*(FPlus + FMinus), R, ERoe, dedp);
+      for (int i = 0; i < NEQ; i++) {
+         add_column(Flux, sumQsp, i, Fc[i*cap + w], R, ERoe, dedp);
+         const uint32_t b = bits[i*cap + w];
+         h = fold_bits(fold_bits(h, b & 15u), b >> 4);
       }
       uint32_t lim_bit = 0;
       if (NS > 1) {
+         const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
+         const double lim_f = dt*(ld<double>(f.m_e[DIR], p) + ld<double>(f.m_e[DIR], pP1));
          bool limit = false;
 #pragma unroll
          for (int i = 0; i < NS; i++) {
             const double dRhoi = lim_f*Flux[i];
             if (((RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i) + dRhoi) < -1e-8) ||
-                ((RK0*ld<double>(f.Conserved_old, P[3], i) + RK1*cons(P[3], i) - dRhoi) < -1e-8))
+                ((RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*cons(pP1, i) - dRhoi) < -1e-8))
                limit = true;
          }
          if (limit) {
             lim_bit = 2;
             // first-order flux -0.5*(FluxP(p) + FluxM(p+1)), back-projected
+            Proj J;
+            proj(J, R);
             double FLO[NEQ];
             double sq = 0.0;
-#pragma unroll 1
+#pragma unroll
             for (int i = 0; i < NEQ; i++) {
-               const double lam = (i == 0) ? Lam1 : (i == NEQ-1) ? LamN : Lam;
+               const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
+               const double Ysel = (i >= 1 && i <= NS) ? R.Y[i-1] : 0.0;
                double ap, am, bp, bm;
-               split(ap, am, i, P[2], un[2], pr[2], lam, R, J, s1q[2], sqq[2], s1f[2], sqf[2]);
-               split(bp, bm, i, P[3], un[3], pr[3], lam, R, J, s1q[3], sqq[3], s1f[3], sqf[3]);
+               split(ap, am, i, p, lam, R, J, Ysel);
+               split(bp, bm, i, pP1, lam, R, J, Ysel);
                add_column(FLO, sq, i, -0.5*(ap + bm), R, ERoe, dedp);
             }
             const double eps = 1e-60;
@@ -339,7 +418,7 @@ struct Euler {
                   theta_p = (fabs(rhoHO_p - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_p - rhoLO)) : 1.0;
                }
                double theta_pp1 = 1.0;
-               const double rhoi_pp1 = RK0*ld<double>(f.Conserved_old, P[3], i) + RK1*cons(P[3], i);
+               const double rhoi_pp1 = RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*cons(pP1, i);
                const double rhoHO_pp1 = rhoi_pp1 - dRhoi;
                if (rhoHO_pp1 < eps) {
                   const double rhoLO = rhoi_pp1 - dRhoi_lo;
@@ -351,18 +430,7 @@ struct Euler {
             for (int i = 0; i < NEQ; i++) Flux[i] = Flux[i]*theta + FLO[i]*(1 - theta);
          }
       }
-      *dec = (h << 2) | lim_bit;
-   }
-
-   // TENO-family face flux for the scheme
-   template<int SCHEME>
-   __device__ void teno_face(double *Flux, const P3 &p, int t, double RK0, double RK1, double dt,
-                             uint32_t *dec) const
-   {
-      const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
-      const double lim_f = dt*(ld<double>(f.m_e[DIR], p) + ld<double>(f.m_e[DIR], pP1));
-      constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
-      teno_flux<OP>(Flux, p, t, RK0, RK1, lim_f, dec);
+      return (h << 2) | lim_bit;
    }
 
    __device__ bool sensor_kg(const P3 &p, int t) const
@@ -373,18 +441,6 @@ struct Euler {
              ld<uint8_t>(f.shockSensor[DIR], pP1);
    }
 
-   // complete face flux (no ring buffer) for the scheme
-   template<int SCHEME>
-   __device__ void face(double *Flux, const P3 &p, double RK0, double RK1, double dt, uint32_t *dec) const
-   {
-      const int t = ld<int>(f.nType[DIR], p);
-      if (SCHEME == HTR_EULER_KG || (SCHEME == HTR_EULER_HYBRID && sensor_kg(p, t))) {
-         kg_flux(Flux, p, t);
-         *dec = 1;
-         return;
-      }
-      teno_face<SCHEME>(Flux, p, t, RK0, RK1, dt, dec);
-   }
 };
 
 // ---- face enumeration --------------------------------------------------------
@@ -469,12 +525,38 @@ euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, d
    }
 }
 
-// Pass 2: TENO-family faces, either all faces or the compacted list.
-template<class MX, int DIR, int SCHEME, bool LISTED>
+// Pass 2: TENO-family faces, either all faces or the compacted list, in
+// three kernels that each keep a small register footprint:
+//   2a  lane per face: Roe state + eigenvalue maxima -> record
+//   2b  lane per (component, face): split fluxes, TENO +/- -> component, bits
+//   2c  lane per face: back-projection, limiter -> face flux, decision word
+// Buffers (capacity cap = number of faces): record NREC*cap doubles, then
+// components NEQ*cap doubles, then bits NEQ*cap bytes.
+struct TenoBufs {
+   double *rec, *Fc;
+   uint8_t *bits;
+   int64_t cap;
+};
+
+template<class MX>
+__host__ __device__ inline int64_t teno_doubles(int64_t cap)
+{ return (Euler<MX, 0>::NREC + MX::NEQ)*cap + (MX::NEQ*cap + 7)/8; }
+
+template<class MX>
+inline TenoBufs teno_bufs(double *base, int64_t cap)
+{
+   TenoBufs b;
+   b.rec = base;
+   b.Fc = base + Euler<MX, 0>::NREC*cap;
+   b.bits = reinterpret_cast<uint8_t*>(b.Fc + MX::NEQ*cap);
+   b.cap = cap;
+   return b;
+}
+
+template<class MX, int DIR, bool LISTED>
 __global__ void __launch_bounds__(256)
-euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double RK0, double RK1,
-                  double dt, double *__restrict__ F, int64_t nfaces, const int32_t *__restrict__ list,
-                  const int *__restrict__ counter, htr_view dec)
+euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, TenoBufs b, int64_t nfaces,
+                 const int32_t *__restrict__ list, const int *__restrict__ counter)
 {
    const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
    const int64_t stride = (int64_t)gridDim.x*blockDim.x;
@@ -482,10 +564,47 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < n; w += stride) {
       const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
       const P3 q = Faces<DIR>::cell(g, f, fid);
-      const int t = ld<int>(f.nType[DIR], q);
+      E.face_record(b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q));
+   }
+}
+
+template<class MX, int DIR, int SCHEME, bool LISTED>
+__global__ void __launch_bounds__(256)
+euler_char_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, TenoBufs b, int64_t nfaces,
+                  const int32_t *__restrict__ list, const int *__restrict__ counter)
+{
+   constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
+   const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
+   const uint32_t nitems = n*(uint32_t)MX::NEQ;
+   const uint32_t stride = gridDim.x*blockDim.x;
+   Euler<MX, DIR> E{f, *mix, g};
+   for (uint32_t it = blockIdx.x*blockDim.x + threadIdx.x; it < nitems; it += stride) {
+      const uint32_t i = it/n;
+      const uint32_t w = it - i*n;
+      const uint32_t fid = LISTED ? (uint32_t)list[w] : w;
+      const P3 q = Faces<DIR>::cell(g, f, fid);
+      uint32_t bits;
+      const double fc = E.template face_component<OP>(b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q), (int)i, &bits);
+      b.Fc[(int64_t)i*b.cap + w] = fc;
+      b.bits[(int64_t)i*b.cap + w] = (uint8_t)bits;
+   }
+}
+
+template<class MX, int DIR, bool LISTED>
+__global__ void __launch_bounds__(256)
+euler_finish_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double RK0, double RK1,
+                    double dt, TenoBufs b, double *__restrict__ F, int64_t nfaces,
+                    const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec)
+{
+   const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
+   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
+   Euler<MX, DIR> E{f, *mix, g};
+   for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < n; w += stride) {
+      const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
+      const P3 q = Faces<DIR>::cell(g, f, fid);
       double Flux[MX::NEQ];
-      uint32_t word;
-      E.template teno_face<SCHEME>(Flux, q, t, RK0, RK1, dt, &word);
+      const uint32_t word = E.face_finish(Flux, b.rec, b.cap, w, b.Fc, b.bits, q, ld<int>(f.nType[DIR], q),
+                                          RK0, RK1, dt);
       store_face<MX::NEQ>(F, nfaces, fid, Flux);
       if (dec.ptr) at<uint32_t>(dec, q) = word;
    }
@@ -512,6 +631,25 @@ face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict
       at<double>(f.Conserved_t, p, c) += (F[c*nfaces + fp] - F[c*nfaces + fm])*m;
 }
 
+template<class MX, int DIR, int SCHEME, bool LISTED>
+static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1, double dt,
+                       Workspace &w, int64_t nfaces, const htr_view &dec, hipStream_t s)
+{
+   constexpr int kMaxBlocks = 256*16;
+   const TenoBufs b = teno_bufs<MX>(w.teno, nfaces);
+   const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxBlocks);
+   const unsigned nbc = (unsigned)std::min<int64_t>(grid_for(nfaces*MX::NEQ, 256), kMaxBlocks);
+   HTR_LAUNCH(HTR_DIR_NAME("euler_roe", DIR), (euler_roe_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
+                      f, mix, g, b, nfaces, w.list, w.counter);
+   int rc = check_launch("euler_roe_kernel"); if (rc) return rc;
+   HTR_LAUNCH(HTR_DIR_NAME("euler_char", DIR), (euler_char_kernel<MX, DIR, SCHEME, LISTED>), dim3(nbc), dim3(256), 0, s,
+                      f, mix, g, b, nfaces, w.list, w.counter);
+   rc = check_launch("euler_char_kernel"); if (rc) return rc;
+   HTR_LAUNCH(HTR_DIR_NAME("euler_finish", DIR), (euler_finish_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
+                      f, mix, g, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec);
+   return check_launch("euler_finish_kernel");
+}
+
 template<class MX, int DIR, int SCHEME>
 static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
                       double dt, const htr_view &dec, hipStream_t s)
@@ -519,31 +657,25 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    const int64_t ncells = exec_cells(g);
    if (ncells == 0) return HTR_OK;
    const int64_t nfaces = Faces<DIR>::count(g);
-   if (nfaces >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many faces");
+   if (nfaces*MX::NEQ >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many faces");
    Workspace &w = workspace();
-   int rc = workspace_faces(w, nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0);
+   int rc = workspace_faces(w, nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0,
+                            SCHEME == HTR_EULER_KG ? 0 : teno_doubles<MX>(nfaces));
    if (rc) return rc;
-   constexpr int kMaxTenoBlocks = 256*8;
    if constexpr (SCHEME == HTR_EULER_KG) {
-      hipLaunchKernelGGL((euler_kg_kernel<MX, DIR, true>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
+      HTR_LAUNCH(HTR_DIR_NAME("euler_kg", DIR), (euler_kg_kernel<MX, DIR, true>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
                          f, mix, g, w.faces, nfaces, w.list, w.counter + 1, dec);
       if ((rc = check_launch("euler_kg_kernel"))) return rc;
    } else if constexpr (SCHEME == HTR_EULER_HYBRID) {
       if ((rc = zero_counters(w.counter, s))) return rc;
-      hipLaunchKernelGGL((euler_kg_kernel<MX, DIR, false>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
+      HTR_LAUNCH(HTR_DIR_NAME("euler_kg", DIR), (euler_kg_kernel<MX, DIR, false>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
                          f, mix, g, w.faces, nfaces, w.list, w.counter + 1, dec);
       if ((rc = check_launch("euler_kg_kernel"))) return rc;
-      const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxTenoBlocks);
-      hipLaunchKernelGGL((euler_teno_kernel<MX, DIR, SCHEME, true>), dim3(nb), dim3(256), 0, s,
-                         f, mix, g, RK0, RK1, dt, w.faces, nfaces, w.list, w.counter, dec);
-      if ((rc = check_launch("euler_teno_kernel"))) return rc;
+      if ((rc = launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
    } else {
-      const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxTenoBlocks);
-      hipLaunchKernelGGL((euler_teno_kernel<MX, DIR, SCHEME, false>), dim3(nb), dim3(256), 0, s,
-                         f, mix, g, RK0, RK1, dt, w.faces, nfaces, w.list, w.counter, dec);
-      if ((rc = check_launch("euler_teno_kernel"))) return rc;
+      if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
    }
-   hipLaunchKernelGGL((face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)), dim3(256), 0, s,
+   HTR_LAUNCH(HTR_DIR_NAME("euler_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)), dim3(256), 0, s,
                       f, g, w.faces, nfaces, f.m_e[DIR], ncells);
    return check_launch("face_divergence_kernel");
 }

@@ -20,13 +20,14 @@ struct Workspace {
    int     *counter;       // device ints: [0] failure counter, [1] face-list length
    double  *faces;         // face fluxes of one sweep, capacity faces_cap doubles
    int32_t *list;          // compacted face indices, capacity list_cap
-   int64_t  faces_cap, list_cap;
+   double  *teno;          // TENO face records, components and bits, capacity teno_cap doubles
+   int64_t  faces_cap, list_cap, teno_cap;
    static constexpr int64_t kPartial = 1 << 16;
 };
 Workspace &workspace();
 // grows the face buffers (grow-only; an allocation happens only when a call
 // needs more than any earlier call on this device)
-int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist);
+int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno);
 
 template<class MX> int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
                                       double RK0, double RK1, double dt, const htr_view &dec, hipStream_t s);
@@ -47,5 +48,19 @@ int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);
 int zero_counters(int *counter, hipStream_t s);   // clears Workspace::counter[0..3]
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1)/block); }
+
+// Optional per-launch timing (htr_profile): when enabled, every launch made
+// through HTR_LAUNCH is bracketed by two HIP events recorded on its stream.
+void prof_begin(hipStream_t s, const char *name);
+void prof_end(hipStream_t s);
+#define HTR_LAUNCH(name, kern, grid, block, shm, s, ...)                      \
+   do {                                                                       \
+      ::htr::prof_begin(s, name);                                             \
+      hipLaunchKernelGGL(kern, grid, block, shm, s, __VA_ARGS__);             \
+      ::htr::prof_end(s);                                                     \
+   } while (0)
+
+// "name_x", "name_y", "name_z"
+#define HTR_DIR_NAME(base, dir) ((dir) == 0 ? base "_x" : (dir) == 1 ? base "_y" : base "_z")
 
 } // namespace htr

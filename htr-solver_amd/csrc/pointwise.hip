@@ -277,7 +277,7 @@ __global__ void zero_counters_kernel(int *counter)
 
 int zero_counters(int *counter, hipStream_t s)
 {
-   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
+   HTR_LAUNCH("zero_counters", zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
    return check_launch("zero_counters_kernel");
 }
 
@@ -285,7 +285,7 @@ int zero_counters(int *counter, hipStream_t s)
 #define HTR_LAUNCH_POINT(kern, ...)                                                       \
    const int64_t n = exec_cells(g);                                                       \
    if (n == 0) return HTR_OK;                                                             \
-   hipLaunchKernelGGL(kern, dim3(grid_for(n, 256)), dim3(256), 0, s, __VA_ARGS__, n);     \
+   HTR_LAUNCH(#kern, kern, dim3(grid_for(n, 256)), dim3(256), 0, s, __VA_ARGS__, n);     \
    return check_launch(#kern);
 
 template<class MX> int properties_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
@@ -330,10 +330,10 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    const int64_t n = exec_cells(g);
    Workspace &w = workspace();
    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(grid_for(n, 256), 1), 4096);
-   hipLaunchKernelGGL(spectral_radius_kernel<MX>, dim3((unsigned)blocks), dim3(256), 0, s, f, mix, g, n, w.partial);
+   HTR_LAUNCH("spectral_radius", spectral_radius_kernel<MX>, dim3((unsigned)blocks), dim3(256), 0, s, f, mix, g, n, w.partial);
    int rc = check_launch("spectral_radius_kernel");
    if (rc) return rc;
-   hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(256), 0, s, w.partial, (int)blocks, w.partial + Workspace::kPartial - 1);
+   HTR_LAUNCH("reduce_max", reduce_max_kernel, dim3(1), dim3(256), 0, s, w.partial, (int)blocks, w.partial + Workspace::kPartial - 1);
    rc = check_launch("reduce_max_kernel");
    if (rc) return rc;
    if (hipMemcpyAsync(w.result_host, w.partial + Workspace::kPartial - 1, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -51,7 +51,7 @@ __device__ inline double ipow12(double x) { const double x6 = ipow6(x); return x
 
 constexpr double C13 = 13.0/12.0, C23 = 3.0/12.0, C14 = 1.0/36.0, C24 = 13.0/12.0, C34 = 781.0/720.0;
 
-__device__ inline void smooth_plus(const double *v, double *s)
+__device__ __forceinline__ void smooth_plus(const double *v, double *s)
 {
    const double ym2 = v[0], ym1 = v[1], y = v[2], yp1 = v[3], yp2 = v[4], yp3 = v[5];
    double a1, a2, a3;
@@ -64,7 +64,7 @@ __device__ inline void smooth_plus(const double *v, double *s)
    s[3] = C14*a1*a1 + C24*a2*a2 + C34*a3*a3;
 }
 
-__device__ inline void smooth_minus(const double *v, double *s)
+__device__ __forceinline__ void smooth_minus(const double *v, double *s)
 {
    const double ym2 = v[0], ym1 = v[1], y = v[2], yp1 = v[3], yp2 = v[4], yp3 = v[5];
    double a1, a2, a3;
@@ -80,7 +80,7 @@ __device__ inline void smooth_minus(const double *v, double *s)
 // OP 0: TENO_Op<-10>, 1: TENOA_Op<-8>, 2: TENOLAD_Op<-16>; SIDE 0 plus, 1 minus.
 // Returns the reconstructed value; *bits receives b1 | b2<<1 | b3<<2 | b4<<3.
 template<int OP, int SIDE>
-__device__ inline double teno_reconstruct(const double *v, int t, uint32_t *bits)
+__device__ __forceinline__ double teno_reconstruct(const double *v, int t, uint32_t *bits)
 {
    const double *Coeffs = SIDE ? c_st.CoeffM[t] : c_st.CoeffP[t];
    const double *Recon  = SIDE ? c_st.ReconM[t] : c_st.ReconP[t];

@@ -264,3 +264,20 @@ def rk_init(fs, nEq, grid=None, stream=None):
 def full_rhs(fs, scheme, RK_coeffs, deltaTime, chem_source, mix, grid=None, stream=None):
     check(lib().htr_full_rhs(_g(fs, grid), C.byref(fs.c()), scheme, RK_coeffs[0], RK_coeffs[1], deltaTime,
                              int(chem_source), mix.h, stream_ptr(stream)))
+
+
+def profile(enable=True):
+    """Per-launch HIP-event timing inside the library (htr_profile)."""
+    check(lib().htr_profile(1 if enable else 0))
+
+
+def profile_report():
+    """{kernel name: (launches, total ms)} of the launches since the last
+    report (waits for them)."""
+    buf = C.create_string_buffer(1 << 16)
+    check(lib().htr_profile_report(buf, len(buf)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, n, ms = line.rsplit(" ", 2)
+        out[name] = (int(n), float(ms))
+    return out

@@ -14,9 +14,18 @@
  *     task's Fluid_bounds argument (periodic wrap is done over it, as
  *     warpPeriodic does), htr_grid.exec is the index space the task runs
  *     over (the domain of its last region argument);
- *   - all pointers are device pointers; no function allocates per call or
- *     synchronises the stream, except htr_max_spectral_radius, which returns
- *     the maximum to the host (the reference returns a future);
+ *   - all FP64 fields of one call share their spatial strides (sx, sy, sz),
+ *     which must be non-negative, with every byte offset inside bounds below
+ *     2^32 per component; vector fields may use any component stride sc;
+ *     the execution rect holds fewer than 2^31 cells (HTR_E_UNSUPPORTED
+ *     otherwise);
+ *   - all pointers are device pointers; nothing synchronises the stream
+ *     except htr_max_spectral_radius, which returns the maximum to the host
+ *     (the reference returns a future), and htr_chem_update, which reads its
+ *     failure count back;
+ *   - device workspace: allocated once per device by htr_init; the Euler
+ *     face buffers grow (only) at the first call on a larger rect, so a
+ *     solver's first step allocates and later steps do not;
  *   - every function returns 0 on success or a negative HTR_E_* code;
  *     htr_last_error() describes the last failure of the calling thread;
  *   - mixtures are uploaded once (htr_mix_*), as LoadMixtureTask does.
@@ -97,6 +106,13 @@ typedef struct htr_mix htr_mix;
 int         htr_init(int device);
 const char *htr_last_error(void);
 const char *htr_version(void);
+
+/* ---- launch timing (no reference counterpart: measurement aid) ------------ */
+/* enable != 0: every kernel launch is bracketed by HIP events on its stream */
+int htr_profile(int enable);
+/* waits for the recorded launches and writes one line per kernel name,
+ * "name launches total_ms", then forgets them */
+int htr_profile_report(char *buf, size_t len);
 
 /* ---- mixtures (src/prometeo_mixture.cu:33-47 LoadMixtureTask) -------------- */
 /* ConstPropMix (src/Mixtures/ConstPropMix.inl:30-50); visc_model 0 constant
