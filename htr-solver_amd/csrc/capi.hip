@@ -254,81 +254,113 @@ static int check_bounds(const htr_grid *g)
    return HTR_OK;
 }
 
-static void rebase(htr_view &v, const int64_t *blo, int esize)
-{
-   if (!v.ptr) return;
-   v.ptr = (char*)v.ptr + (blo[0]*v.sx + blo[1]*v.sy + blo[2]*v.sz)*esize;
-}
+static int64_t gcd64(int64_t a, int64_t b) { while (b) { const int64_t t = a % b; a = b; b = t; } return a; }
 
-// Normalises the caller's views for the kernels (see common.hpp): every view
-// rebased to the low corner of Fluid_bounds; all FP64 fields must share their
-// spatial element strides, non-negative, with every byte offset in the
-// bounds below 2^32; the execution rect holds fewer than 2^31 cells.
-static int layout(const htr_grid *g, const htr_fields *in, htr_fields &out, Geo &geo)
+// Normalises the caller's views for the kernels (see common.hpp).  Every
+// view's spatial strides, over the directions of extent > 1, must be an
+// integer multiple k >= 1 of one common triple; strides are non-negative,
+// every byte offset inside the bounds is below 2^32, and the execution rect
+// holds fewer than 2^31 cells.
+struct LayoutCheck {
+   int64_t bsz[3];
+   int64_t base[3];
+   bool have = false;
+   const char *first = "";
+   int add(htr_view &v, int esize, const char *name, const int64_t *blo)
+   {
+      if (!v.ptr) return HTR_OK;
+      int64_t s[3] = { v.sx, v.sy, v.sz };
+      int64_t k = 0;
+      for (int d = 0; d < 3; d++) {
+         if (bsz[d] == 1) s[d] = 0;
+         if (s[d] < 0) return set_error(HTR_E_UNSUPPORTED, "field %s: negative stride", name);
+         k = gcd64(k, s[d]);
+      }
+      if (k == 0) k = 1;
+      int64_t b[3];
+      for (int d = 0; d < 3; d++) b[d] = s[d]/k;
+      if (!have) {
+         for (int d = 0; d < 3; d++) base[d] = b[d];
+         have = true;
+         first = name;
+      } else {
+         for (int d = 0; d < 3; d++)
+            if (b[d] != base[d])
+               return set_error(HTR_E_UNSUPPORTED, "fields %s and %s: spatial strides are not multiples of one "
+                                "stride triple", first, name);
+      }
+      int64_t span = 0;
+      for (int d = 0; d < 3; d++) span += (bsz[d] - 1)*s[d];
+      if (span*esize >= (1ll << 32))
+         return set_error(HTR_E_UNSUPPORTED, "field %s spans 4 GiB or more per component", name);
+      v.ptr = (char*)v.ptr + (blo[0]*v.sx + blo[1]*v.sy + blo[2]*v.sz)*esize;
+      v.sx = k*esize;
+      v.sy = v.sz = 0;
+      return HTR_OK;
+   }
+};
+
+static int layout_views(const htr_grid *g, htr_fields &out, htr_view *extra, int nextra, const int *extra_esize,
+                        Geo &geo)
 {
-   int rc = check_bounds(g); if (rc) return rc;
-   out = *in;
    const int64_t blo[3] = { g->bounds.lo[0], g->bounds.lo[1], g->bounds.lo[2] };
-   int64_t bsz[3], esz[3];
+   LayoutCheck L;
+   int64_t esz[3];
    for (int d = 0; d < 3; d++) {
-      bsz[d] = g->bounds.hi[d] - g->bounds.lo[d] + 1;
+      L.bsz[d] = g->bounds.hi[d] - g->bounds.lo[d] + 1;
       esz[d] = g->exec.hi[d] - g->exec.lo[d] + 1;
-      geo.bsz[d] = (int32_t)bsz[d];
+      geo.bsz[d] = (int32_t)L.bsz[d];
       geo.elo[d] = (int32_t)(g->exec.lo[d] - blo[d]);
       geo.esz[d] = (int32_t)esz[d];
    }
    if (esz[0]*esz[1]*esz[2] >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "execution rect holds 2^31 cells or more");
-   struct Named { htr_view *v; const char *name; };
-   const Named dbl[] = {
-      {&out.m_e[0], "m_e[0]"}, {&out.m_e[1], "m_e[1]"}, {&out.m_e[2], "m_e[2]"},
-      {&out.m_d[0], "m_d[0]"}, {&out.m_d[1], "m_d[1]"}, {&out.m_d[2], "m_d[2]"},
-      {&out.m_s[0], "m_s[0]"}, {&out.m_s[1], "m_s[1]"}, {&out.m_s[2], "m_s[2]"},
-      {&out.rho, "rho"}, {&out.mu, "mu"}, {&out.lam, "lam"}, {&out.Di, "Di"}, {&out.SoS, "SoS"},
-      {&out.pressure, "pressure"}, {&out.temperature, "temperature"}, {&out.MassFracs, "MassFracs"},
-      {&out.MolarFracs, "MolarFracs"}, {&out.velocity, "velocity"}, {&out.Conserved, "Conserved"},
-      {&out.Conserved_old, "Conserved_old"}, {&out.Conserved_t, "Conserved_t"},
-      {&out.Conserved_t_old, "Conserved_t_old"}, {&out.DucrosSensor, "DucrosSensor"} };
-   const htr_view *ref = nullptr;
-   const char *refname = "";
-   for (const Named &n : dbl) {
-      if (!n.v->ptr) continue;
-      if (!ref) { ref = n.v; refname = n.name; continue; }
-      if (n.v->sx != ref->sx || n.v->sy != ref->sy || n.v->sz != ref->sz)
-         return set_error(HTR_E_UNSUPPORTED, "FP64 fields %s and %s have different spatial strides", refname, n.name);
+   struct Named { htr_view *v; int esize; const char *name; };
+   const Named all[] = {
+      {&out.m_e[0], 8, "m_e[0]"}, {&out.m_e[1], 8, "m_e[1]"}, {&out.m_e[2], 8, "m_e[2]"},
+      {&out.m_d[0], 8, "m_d[0]"}, {&out.m_d[1], 8, "m_d[1]"}, {&out.m_d[2], 8, "m_d[2]"},
+      {&out.m_s[0], 8, "m_s[0]"}, {&out.m_s[1], 8, "m_s[1]"}, {&out.m_s[2], 8, "m_s[2]"},
+      {&out.rho, 8, "rho"}, {&out.mu, 8, "mu"}, {&out.lam, 8, "lam"}, {&out.Di, 8, "Di"}, {&out.SoS, 8, "SoS"},
+      {&out.pressure, 8, "pressure"}, {&out.temperature, 8, "temperature"}, {&out.MassFracs, 8, "MassFracs"},
+      {&out.MolarFracs, 8, "MolarFracs"}, {&out.velocity, 8, "velocity"}, {&out.Conserved, 8, "Conserved"},
+      {&out.Conserved_old, 8, "Conserved_old"}, {&out.Conserved_t, 8, "Conserved_t"},
+      {&out.Conserved_t_old, 8, "Conserved_t_old"}, {&out.DucrosSensor, 8, "DucrosSensor"},
+      {&out.nType[0], 4, "nType[0]"}, {&out.nType[1], 4, "nType[1]"}, {&out.nType[2], 4, "nType[2]"},
+      {&out.shockSensor[0], 1, "shockSensor[0]"}, {&out.shockSensor[1], 1, "shockSensor[1]"},
+      {&out.shockSensor[2], 1, "shockSensor[2]"} };
+   for (const Named &n : all) {
+      const int rc = L.add(*n.v, n.esize, n.name, blo);
+      if (rc) return rc;
    }
-   int64_t s[3] = {0, 0, 0};
-   if (ref) { s[0] = ref->sx; s[1] = ref->sy; s[2] = ref->sz; }
-   int64_t span = 0;
-   for (int d = 0; d < 3; d++) {
-      if (s[d] < 0) return set_error(HTR_E_UNSUPPORTED, "negative strides");
-      span += (bsz[d] - 1)*s[d]*8;
-      geo.sb[d] = (uint32_t)(s[d]*8);
+   for (int i = 0; i < nextra; i++) {
+      const int rc = L.add(extra[i], extra_esize[i], "diagnostics", blo);
+      if (rc) return rc;
    }
-   if (span >= (1ll << 32)) return set_error(HTR_E_UNSUPPORTED, "FP64 fields span 4 GiB or more per component");
-   for (const Named &n : dbl) rebase(*n.v, blo, 8);
-   for (int d = 0; d < 3; d++) { rebase(out.nType[d], blo, 4); rebase(out.shockSensor[d], blo, 1); }
+   for (int d = 0; d < 3; d++) geo.cb[d] = (uint32_t)(L.have ? L.base[d] : 0);
    return HTR_OK;
 }
 
-static int prep_plain(const htr_grid *g, const htr_fields *f, htr_fields &fr, Geo &geo)
+static int layout(const htr_grid *g, const htr_fields *in, htr_fields &out, Geo &geo,
+                  htr_view *extra = nullptr, int nextra = 0, const int *extra_esize = nullptr)
+{
+   int rc = check_bounds(g); if (rc) return rc;
+   out = *in;
+   return layout_views(g, out, extra, nextra, extra_esize, geo);
+}
+
+static int prep_plain(const htr_grid *g, const htr_fields *f, htr_fields &fr, Geo &geo,
+                      htr_view *extra = nullptr, int nextra = 0, const int *extra_esize = nullptr)
 {
    if (!g || !f) return set_error(HTR_E_ARG, "null argument");
-   int rc = layout(g, f, fr, geo); if (rc) return rc;
+   int rc = layout(g, f, fr, geo, extra, nextra, extra_esize); if (rc) return rc;
    return ensure_device();
 }
 
-static int prep(const htr_grid *g, const htr_fields *f, const htr_mix *m, htr_fields &fr, Geo &geo)
+static int prep(const htr_grid *g, const htr_fields *f, const htr_mix *m, htr_fields &fr, Geo &geo,
+                htr_view *extra = nullptr, int nextra = 0, const int *extra_esize = nullptr)
 {
    if (!m) return set_error(HTR_E_ARG, "null argument");
    if (!m->finalized) return set_error(HTR_E_ARG, "mixture not finalized");
-   return prep_plain(g, f, fr, geo);
-}
-
-static htr_view rebased(const htr_grid *g, htr_view v, int esize)
-{
-   const int64_t blo[3] = { g->bounds.lo[0], g->bounds.lo[1], g->bounds.lo[2] };
-   rebase(v, blo, esize);
-   return v;
+   return prep_plain(g, f, fr, geo, extra, nextra, extra_esize);
 }
 
 static int need(const htr_view &v, const char *name)
@@ -558,15 +590,16 @@ int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
                    double RK0, double RK1, double deltaTime, const htr_mix *m,
                    const htr_diag *diag, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   htr_view dec = {};
+   if (diag && dir >= 0 && dir <= 2) dec = diag->decisions[dir];
+   const int esz4 = 4;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &dec, 1, &esz4); if (rc) return rc;
    if (dir < 0 || dir > 2) return set_error(HTR_E_ARG, "bad direction %d", dir);
    if (scheme < HTR_EULER_HYBRID || scheme > HTR_EULER_KG) return set_error(HTR_E_ARG, "bad scheme %d", scheme);
    HTR_NEED(Conserved); HTR_NEED(rho); HTR_NEED(MassFracs); HTR_NEED(velocity); HTR_NEED(pressure);
    HTR_NEED(nType[dir]); HTR_NEED(m_e[dir]); HTR_NEED(Conserved_t);
    if (scheme != HTR_EULER_KG) { HTR_NEED(SoS); if (m->host.nSpec > 1) HTR_NEED(Conserved_old); }
    if (scheme == HTR_EULER_HYBRID) HTR_NEED(shockSensor[dir]);
-   htr_view dec = {};
-   if (diag) dec = rebased(g, diag->decisions[dir], 4);
    HTR_DISPATCH3(euler_dispatch, dir, scheme, fr, m->dev, geo, RK0, RK1, deltaTime, dec, (hipStream_t)stream);
 }
 
@@ -597,11 +630,12 @@ int htr_chem_source(const htr_grid *g, const htr_fields *f, const htr_mix *m, vo
 int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int precision,
                     const htr_mix *m, const htr_diag *diag, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   htr_view ns = {};
+   if (diag) ns = diag->nsteps;
+   const int esz4 = 4;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &ns, 1, &esz4); if (rc) return rc;
    if (precision != HTR_CHEM_FP64 && precision != HTR_CHEM_FP32_IR) return set_error(HTR_E_ARG, "bad precision %d", precision);
    HTR_NEED(Conserved); HTR_NEED(Conserved_t); HTR_NEED(Conserved_t_old); HTR_NEED(temperature);
-   htr_view ns = {};
-   if (diag) ns = rebased(g, diag->nsteps, 4);
    switch (kind_of(m)) {
       case K_AIR: return chem_update_launch<MixAir>(fr, m->dev, geo, dt, precision, ns, (hipStream_t)stream);
       default: return set_error(HTR_E_UNSUPPORTED, "implicit chemistry: no kernel instance for %d species", m->host.nSpec);

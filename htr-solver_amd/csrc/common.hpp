@@ -2,12 +2,16 @@
 // the HTR hot-path kernels.
 //
 // Device-side addressing.  The host rebases every view to the low corner of
-// Fluid_bounds, so kernel coordinates run over [0, bsz) in each direction,
-// and checks that all FP64 cell fields share one set of spatial strides.  A
-// point then carries a single 32-bit byte offset that is valid for every
-// FP64 field: a load is (uniform field base + component offset) + (per-lane
-// 32-bit offset), which the backend emits as a global load with a scalar
-// base and one VGPR of offset instead of a 64-bit address per field.
+// Fluid_bounds (kernel coordinates run over [0, bsz) in each direction) and
+// checks that the spatial strides of every view are an integer multiple k of
+// one common triple (cx, cy, cz): that covers structure-of-arrays fields
+// (k = 1, components sc = N apart) as well as the reference's per-point
+// arrays (VecNEq, VecNSp, Vec3: k = number of components, sc = 1).  A point
+// carries one 32-bit offset o = x cx + y cy + z cz, and an access is
+//   field base + comp*sc*sizeof(T)  (uniform)  +  o * k*sizeof(T)  (32-bit),
+// so the backend uses a scalar base and one VGPR of offset per access instead
+// of a 64-bit address per (point, field).  In the kernels' copy of a view,
+// sx holds k*sizeof(T) in bytes; sy and sz are unused.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,36 +26,33 @@ struct Geo {
    int32_t  bsz[3];   // Fluid_bounds extent
    int32_t  elo[3];   // execution rect, low corner
    int32_t  esz[3];   // execution rect, extent
-   uint32_t sb[3];    // byte strides shared by the FP64 cell fields
+   uint32_t cb[3];    // common stride triple (cx, cy, cz)
 };
 
 struct P3 {
    int32_t  c[3];
-   uint32_t o;        // byte offset of the point in the FP64 fields
+   uint32_t o;        // x cx + y cy + z cz
 };
 
 __device__ inline P3 mkp(const Geo &g, int x, int y, int z)
 {
    P3 p;
    p.c[0] = x; p.c[1] = y; p.c[2] = z;
-   p.o = (uint32_t)x*g.sb[0] + (uint32_t)y*g.sb[1] + (uint32_t)z*g.sb[2];
+   p.o = (uint32_t)x*g.cb[0] + (uint32_t)y*g.cb[1] + (uint32_t)z*g.cb[2];
    return p;
 }
 
 __device__ inline void setc(const Geo &g, P3 &p, int d, int v)
 {
-   p.o += (uint32_t)(v - p.c[d])*g.sb[d];
+   p.o += (uint32_t)(v - p.c[d])*g.cb[d];
    p.c[d] = v;
 }
 
 template<typename T>
 __device__ inline const T *lp(const htr_view &v, const P3 &p, int comp)
 {
-   if constexpr (sizeof(T) == 8)
-      return reinterpret_cast<const T*>(reinterpret_cast<const char*>(v.ptr) + comp*v.sc*8 + p.o);
-   else
-      return reinterpret_cast<const T*>(v.ptr) + ((int64_t)p.c[0]*v.sx + (int64_t)p.c[1]*v.sy +
-                                                   (int64_t)p.c[2]*v.sz + comp*v.sc);
+   return reinterpret_cast<const T*>(reinterpret_cast<const char*>(v.ptr) + comp*v.sc*(int64_t)sizeof(T) +
+                                     p.o*(uint32_t)v.sx);
 }
 
 template<typename T>

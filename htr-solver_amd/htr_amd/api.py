@@ -20,7 +20,7 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "update_using_euler_flux", "update_using_hybrid_euler_flux", "update_using_teno_euler_flux",
            "update_using_skew_symmetric_euler_flux", "update_using_diffusion_flux",
            "add_chemistry_sources", "update_chemistry", "calculate_max_spectral_radius",
-           "rk_update", "rk_init", "full_rhs", "stream_ptr"]
+           "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report"]
 
 
 def stream_ptr(stream=None):
@@ -117,10 +117,14 @@ VECTORS = {"Di": "ns", "MassFracs": "ns", "MolarFracs": "ns", "velocity": 3,
 
 
 class FieldSet:
-    """Device fields of one tile in structure-of-arrays layout: a field is a
-    (ncomp, nz, ny, nx) tensor, x fastest, components far apart."""
+    """Device fields of one tile.  Every vector field is presented as a
+    (ncomp, nz, ny, nx) tensor, x fastest.  layout "soa": components far apart
+    (one block per component); "aos": the reference's per-point arrays
+    (VecNEq, VecNSp, Vec3 of src/prometeo_types.h), stored (nz, ny, nx, ncomp)
+    and presented through a permuted view."""
 
-    def __init__(self, n, nspec, device="cuda", bounds_lo=(0, 0, 0), decisions=False, nsteps=False):
+    def __init__(self, n, nspec, device="cuda", bounds_lo=(0, 0, 0), decisions=False, nsteps=False,
+                 layout="soa"):
         nx, ny, nz = n
         self.n = tuple(n)
         self.ns = nspec
@@ -131,7 +135,10 @@ class FieldSet:
             self.t[k] = torch.zeros((nz, ny, nx), dtype=torch.float64, **kw)
         for k, c in VECTORS.items():
             nc = {"ns": nspec, "neq": nspec + 4}.get(c, c)
-            self.t[k] = torch.zeros((nc, nz, ny, nx), dtype=torch.float64, **kw)
+            if layout == "aos":
+                self.t[k] = torch.zeros((nz, ny, nx, nc), dtype=torch.float64, **kw).permute(3, 0, 1, 2)
+            else:
+                self.t[k] = torch.zeros((nc, nz, ny, nx), dtype=torch.float64, **kw)
         for d in range(3):
             self.t["nType%d" % d] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
             for k in ("m_e", "m_d", "m_s"):

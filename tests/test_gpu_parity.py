@@ -221,14 +221,17 @@ def test_chem_source_and_cfl(air16):
     assert abs(r_g - r_o) <= 1e-13 * abs(r_o)
 
 
-def test_full_rhs_air(air16):
+@pytest.mark.parametrize("layout", ["soa", "aos"])
+def test_full_rhs_air(air16, layout):
+    """Full RHS; "aos" stores every vector field as the reference does (one
+    contiguous array of components per point), "soa" one block per component."""
     import htr_amd as H
     st, meta, n, om = air16
-    say("case air16 full RHS (source + Euler ZYX + diffusion ZYX)")
+    say("case air16 full RHS (source + Euler ZYX + diffusion ZYX), layout", layout)
     dt = dt_cfl(st, meta, n)
     g, f = oracle_fields(st, 5, n)
     O.full_rhs(g, f, O.SCHEME_HYBRID, 1.0, 0.0, dt, 1, om)
-    fs = to_gpu(st, 5, n)
+    fs = to_gpu(st, 5, n, layout=layout)
     H.full_rhs(fs, 0, (1.0, 0.0), dt, True, gpu_mix(meta))
     e = norm_err(host(fs["Conserved_t"]), f["Conserved_t"])
     say("  full RHS: %s" % ["%.2e" % x for x in e])
