@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB FP64 tensor 5 times (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
@@ -180,8 +182,31 @@ def units_of(fam, name, n, counts):
     return cells
 
 
+def calibrate_copy():
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
+    this kernel, divided by its 1 GiB per launch, scale the counters of the hot
+    kernels to bytes."""
+    import torch
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5,
+                      "ms_per_copy": ms, "GBps_read_plus_write": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's share of the "
               "domain is the unit of work; see the sharding section of DESIGN.md)", file=sys.stderr)

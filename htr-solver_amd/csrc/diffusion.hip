@@ -2,9 +2,9 @@
 // velocity) flux divergence along one direction.
 //
 // Reference: src/prometeo_rhs.inl:1058-1282 (GetSigma, GetDiffusionFlux,
-// updateRHSSpan), src/prometeo_rhs.cu:461-622.  Same segment walk as the Euler
-// kernels; the transverse velocity gradients of the face's right cell are
-// carried to the next face, as the reference does (rhs.inl:1212-1214).
+// updateRHSSpan), src/prometeo_rhs.cu:461-622.  Two passes per direction, as
+// for the Euler fluxes: every face flux into the face buffer (one lane per
+// face, x-fastest), then the coalesced divergence with m_d.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -109,52 +109,60 @@ struct Diffusion {
    }
 };
 
-template<class MX, int DIR, int SEG>
+// One lane per face (the face enumeration of the Euler passes): face j = 0
+// is the minus face of the line's first cell p0, at pm1 = p0 + offM1(nType);
+// face j >= 1 is the plus face of cell elo + j - 1.  The transverse velocity
+// gradients entering a face's left value are those the reference carries from
+// the previous face's right cell (rhs.inl:1212-1214): the face cell itself,
+// except for j = 1, where they belong to pm1 + 1 (different from p0 only when
+// offM1(nType[p0]) = 0, a staggered wall node).
+template<class MX, int DIR>
 __global__ void __launch_bounds__(256)
-diffusion_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t nitems)
+diffusion_face_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ F,
+                      int64_t nfaces)
 {
-   const int64_t tid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
-   if (tid >= nitems) return;
-   constexpr int A = (DIR == 0) ? 1 : 0;
-   constexpr int B = (DIR == 2) ? 1 : 2;
-   uint32_t r = (uint32_t)tid;
+   const int64_t fid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+   if (fid >= nfaces) return;
+   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B;
+   uint32_t line; int j;
+   Faces<DIR>::split(g, (uint32_t)fid, line, j);
    int c[3];
-   c[A] = g.elo[A] + (int)(r % (uint32_t)g.esz[A]); r /= (uint32_t)g.esz[A];
-   c[B] = g.elo[B] + (int)(r % (uint32_t)g.esz[B]); r /= (uint32_t)g.esz[B];
-   const int s0 = (int)r*SEG;
-   const int s1 = min(s0 + SEG, g.esz[DIR]);
-   c[DIR] = g.elo[DIR] + s0;
-   P3 p = mkp(g, c[0], c[1], c[2]);
-   Diffusion<MX, DIR> D{f, *mix, g};
-   constexpr int NEQ = MX::NEQ;
-   double FM[NEQ], FP[NEQ], g1[2][3], g2[2][3];
-   {
-      const P3 q = wminus(g, DIR, p, c_st.Cp[ld<int>(f.nType[DIR], p)][1]);
-      D.template deriv3<Diffusion<MX, DIR>::D1>(q, g1[0]);
-      D.template deriv3<Diffusion<MX, DIR>::D2>(q, g2[0]);
-      D.flux(FM, q, g1, g2);
+   c[A] = g.elo[A] + (int)(line % (uint32_t)g.esz[A]);
+   c[B] = g.elo[B] + (int)(line / (uint32_t)g.esz[A]);
+   c[DIR] = g.elo[DIR];
+   const P3 p0 = mkp(g, c[0], c[1], c[2]);
+   const P3 pm1 = wminus(g, DIR, p0, c_st.Cp[ld<int>(f.nType[DIR], p0)][1]);
+   P3 cell = pm1, gl = pm1;
+   if (j > 0) {
+      cell = p0;
+      setc(g, cell, DIR, g.elo[DIR] + j - 1);
+      gl = (j == 1) ? wplus(g, DIR, pm1, 1) : cell;
    }
-   for (int s = s0; s < s1; s++) {
-      setc(g, p, DIR, g.elo[DIR] + s);
-      D.flux(FP, p, g1, g2);
-      const double md = ld<double>(f.m_d[DIR], p);
-#pragma unroll
-      for (int i = 0; i < NEQ; i++) {
-         at<double>(f.Conserved_t, p, i) += (FP[i] - FM[i])*md;
-         FM[i] = FP[i];
-      }
-   }
+   using D_t = Diffusion<MX, DIR>;
+   D_t D{f, *mix, g};
+   double g1[2][3], g2[2][3], Flux[MX::NEQ];
+   D.template deriv3<D_t::D1>(gl, g1[0]);
+   D.template deriv3<D_t::D2>(gl, g2[0]);
+   D.flux(Flux, cell, g1, g2);
+   store_face<MX::NEQ>(F, nfaces, (uint32_t)fid, Flux);
 }
 
 template<class MX, int DIR>
 static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
 {
-   constexpr int SEG = 16;
-   const int64_t nseg = (g.esz[DIR] + SEG - 1)/SEG;
-   const int64_t n = exec_cells(g)/g.esz[DIR]*nseg;
-   if (n == 0) return HTR_OK;
-   HTR_LAUNCH(HTR_DIR_NAME("diffusion", DIR), (diffusion_kernel<MX, DIR, SEG>), dim3(grid_for(n, 256)), dim3(256), 0, s, f, mix, g, n);
-   return check_launch("diffusion_kernel");
+   const int64_t ncells = exec_cells(g);
+   if (ncells == 0) return HTR_OK;
+   const int64_t nfaces = Faces<DIR>::count(g);
+   if (nfaces*MX::NEQ >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many faces");
+   Workspace &w = workspace();
+   int rc = workspace_faces(w, nfaces*MX::NEQ, 0, 0);
+   if (rc) return rc;
+   HTR_LAUNCH(HTR_DIR_NAME("diffusion_face", DIR), (diffusion_face_kernel<MX, DIR>), dim3(grid_for(nfaces, 256)),
+              dim3(256), 0, s, f, mix, g, w.faces, nfaces);
+   if ((rc = check_launch("diffusion_face_kernel"))) return rc;
+   HTR_LAUNCH(HTR_DIR_NAME("diffusion_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>),
+              dim3(grid_for(ncells, 256)), dim3(256), 0, s, f, g, w.faces, nfaces, f.m_d[DIR], ncells);
+   return check_launch("face_divergence_kernel");
 }
 
 template<class MX>
