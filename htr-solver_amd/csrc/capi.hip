@@ -638,8 +638,14 @@ int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int preci
    HTR_NEED(Conserved); HTR_NEED(Conserved_t); HTR_NEED(Conserved_t_old); HTR_NEED(temperature);
    switch (kind_of(m)) {
       case K_AIR: return chem_update_launch<MixAir>(fr, m->dev, geo, dt, precision, ns, (hipStream_t)stream);
+      case K_30:  return chem_update_launch<Mix30>(fr, m->dev, geo, dt, precision, ns, (hipStream_t)stream);
       default: return set_error(HTR_E_UNSUPPORTED, "implicit chemistry: no kernel instance for %d species", m->host.nSpec);
    }
+}
+
+int htr_chem_launch_config(int block, int blocks_per_cu)
+{
+   return chem_set_launch(block, blocks_per_cu);
 }
 
 int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mix *m,

@@ -8,11 +8,15 @@
 // Two linear-solve precisions:
 //   HTR_CHEM_FP64     Crout LU with implicit scaling and partial pivoting in
 //                     FP64, as the reference;
-//   HTR_CHEM_FP32_IR  the same LU factorised in FP32, each of the four stage
-//                     solves recovered to FP64 accuracy by residual-based
-//                     iterative refinement against the FP64 matrix
-//                     (r = b - A x in FP64, x += LU32 \ r) until the
-//                     correction is below 1e-15 relative or 4 sweeps.
+//   HTR_CHEM_FP32_IR  the matrix equilibrated by powers of two and factorised
+//                     in FP32; each of the four stage solves is recovered to
+//                     FP64 accuracy by residual-based iterative refinement
+//                     against the FP64 matrix (r = b - A x in FP64,
+//                     x += LU32 \ r) until the correction is below 1e-15 of
+//                     the solution; if a sweep fails to halve the correction
+//                     the solve falls back to an FP64 LU of the same matrix
+//                     (made once per factorisation), so the result always has
+//                     FP64 accuracy.
 //
 // One thread integrates one cell (cells of a wave march independently; the
 // step count varies per cell).  The cell loop is a grid-stride loop over a
@@ -118,9 +122,13 @@ struct Rosen {
    }
 
    double Jac[N*N];
-   double A64[PREC == HTR_CHEM_FP64 ? N*N : N*N];   // FP64: the LU itself; IR: the unfactorised matrix
-   float  A32[PREC == HTR_CHEM_FP64 ? 1 : N*N];
-   int    ind[N];
+   double A64[N*N];                                   // FP64: the LU itself; IR: the unfactorised matrix
+   float  A32[PREC == HTR_CHEM_FP64 ? 1 : N*N];       // IR: the scaled FP32 LU
+   double L64[PREC == HTR_CHEM_FP64 ? 1 : N*N];       // IR: FP64 LU made only when refinement stalls
+   double rs[PREC == HTR_CHEM_FP64 ? 1 : N], cs[PREC == HTR_CHEM_FP64 ? 1 : N];
+   int    ind[N], ind64[PREC == HTR_CHEM_FP64 ? 1 : N];
+   bool   lu64_ready;
+   int    fallbacks = 0;
    double err[N], dx[N], xsav[N], dxsav[N];
 
    __device__ void jacobian(double *x)
@@ -141,6 +149,15 @@ struct Rosen {
       }
    }
 
+   // power of two nearest to 1/a (exact scaling)
+   __device__ static double pow2_inv(double a)
+   {
+      if (!(a > 0.0) || !(a < 1.0e308)) return 1.0;
+      int e;
+      frexp(a, &e);
+      return ldexp(1.0, -e);
+   }
+
    __device__ void factor(double dt)
    {
       for (int j = 0; j < N; j++)
@@ -149,35 +166,74 @@ struct Rosen {
       if (PREC == HTR_CHEM_FP64) {
          ludcmp<double>(A64, ind);
       } else {
-         for (int k = 0; k < N*N; k++) A32[k] = (float)A64[k];
+         // equilibrate by powers of two (rows, then columns) before rounding to
+         // FP32: the FP32 LU then sees entries of comparable magnitude
+         for (int i = 0; i < N; i++) {
+            double big = 0.0;
+            for (int j = 0; j < N; j++) big = dmax(big, fabs(A64[i + j*N]));
+            rs[i] = pow2_inv(big);
+         }
+         for (int j = 0; j < N; j++) {
+            double big = 0.0;
+            for (int i = 0; i < N; i++) big = dmax(big, fabs(rs[i]*A64[i + j*N]));
+            cs[j] = pow2_inv(big);
+         }
+         for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) A32[i + j*N] = (float)(rs[i]*A64[i + j*N]*cs[j]);
          ludcmp<float>(A32, ind);
+         lu64_ready = false;
       }
+   }
+
+   // x = A^-1 r through the scaled FP32 LU
+   __device__ void solve32(const double *r, double *x)
+   {
+      float c[N];
+#pragma unroll
+      for (int i = 0; i < N; i++) c[i] = (float)(rs[i]*r[i]);
+      lubksb<float, float>(A32, ind, c);
+#pragma unroll
+      for (int i = 0; i < N; i++) x[i] = cs[i]*(double)c[i];
    }
 
    __device__ void solve(double *b)
    {
       if (PREC == HTR_CHEM_FP64) { lubksb<double, double>(A64, ind, b); return; }
-      // FP32 solve + FP64 residual refinement
-      double rhs0[N], x[N];
-      float  c[N];
+      // FP32 solve + FP64 residual refinement, until the correction is below
+      // 1e-15 of the solution; a stalled refinement (the correction shrinks by
+      // less than half in a sweep) falls back to an FP64 LU of the same matrix
+      double rhs0[N], x[N], r[N], c[N];
 #pragma unroll
-      for (int i = 0; i < N; i++) { rhs0[i] = b[i]; c[i] = (float)b[i]; }
-      lubksb<float, float>(A32, ind, c);
-#pragma unroll
-      for (int i = 0; i < N; i++) x[i] = (double)c[i];
-      for (int it = 0; it < 4; it++) {
+      for (int i = 0; i < N; i++) rhs0[i] = b[i];
+      solve32(rhs0, x);
+      bool done = false;
+      double prev = 0.0;
+      for (int it = 0; it < 8 && !done; it++) {
          double nx = 0.0;
          for (int i = 0; i < N; i++) {
-            double r = rhs0[i];
-            for (int j = 0; j < N; j++) r -= A64[i + j*N]*x[j];
-            c[i] = (float)r;
+            double s = rhs0[i];
+            for (int j = 0; j < N; j++) s -= A64[i + j*N]*x[j];
+            r[i] = s;
             nx = dmax(nx, fabs(x[i]));
          }
-         lubksb<float, float>(A32, ind, c);
+         solve32(r, c);
          double nc = 0.0;
 #pragma unroll
-         for (int i = 0; i < N; i++) { x[i] += (double)c[i]; nc = dmax(nc, fabs((double)c[i])); }
-         if (nc <= 1e-15*nx) break;
+         for (int i = 0; i < N; i++) { x[i] += c[i]; nc = dmax(nc, fabs(c[i])); }
+         if (!(nc > 1e-15*nx)) done = true;
+         else if (it > 0 && !(nc < 0.5*prev)) break;
+         prev = nc;
+      }
+      if (!done) {
+         if (!lu64_ready) {
+            for (int k = 0; k < N*N; k++) L64[k] = A64[k];
+            ludcmp<double>(L64, ind64);
+            lu64_ready = true;
+         }
+#pragma unroll
+         for (int i = 0; i < N; i++) x[i] = rhs0[i];
+         lubksb<double, double>(L64, ind64, x);
+         fallbacks++;
       }
 #pragma unroll
       for (int i = 0; i < N; i++) b[i] = x[i];
@@ -261,7 +317,7 @@ struct Rosen {
 };
 
 template<class MX, int PREC>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 chem_update_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double dt,
                    htr_view nsteps, int *__restrict__ fails, int64_t n)
 {
@@ -285,6 +341,21 @@ chem_update_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
    }
 }
 
+// Launch shape of the persistent cell loop: threads per block and resident
+// blocks per CU (256 CUs); set by htr_chem_launch_config, chosen by the
+// autotuner in htr_amd.chem_autotune.
+static int g_chem_block = 64;
+static int g_chem_blocks_per_cu = 16;
+
+int chem_set_launch(int block, int blocks_per_cu)
+{
+   if (block != 64 && block != 128 && block != 256) return set_error(HTR_E_ARG, "chem block size %d", block);
+   if (blocks_per_cu < 1 || blocks_per_cu > 64) return set_error(HTR_E_ARG, "chem blocks per CU %d", blocks_per_cu);
+   g_chem_block = block;
+   g_chem_blocks_per_cu = blocks_per_cu;
+   return HTR_OK;
+}
+
 template<class MX>
 int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, double dt, int precision,
                        const htr_view &nsteps, hipStream_t s)
@@ -293,11 +364,12 @@ int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, doubl
    if (n == 0) return HTR_OK;
    Workspace &w = workspace();
    if (int rc0 = zero_counters(w.counter, s)) return rc0;
-   const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, 64), 256*16);
+   const int bs = g_chem_block;
+   const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, bs), 256*g_chem_blocks_per_cu);
    if (precision == HTR_CHEM_FP64)
-      HTR_LAUNCH("chem_update_fp64", (chem_update_kernel<MX, HTR_CHEM_FP64>), dim3(blocks), dim3(64), 0, s, f, mix, g, dt, nsteps, w.counter, n);
+      HTR_LAUNCH("chem_update_fp64", (chem_update_kernel<MX, HTR_CHEM_FP64>), dim3(blocks), dim3(bs), 0, s, f, mix, g, dt, nsteps, w.counter, n);
    else
-      HTR_LAUNCH("chem_update_fp32ir", (chem_update_kernel<MX, HTR_CHEM_FP32_IR>), dim3(blocks), dim3(64), 0, s, f, mix, g, dt, nsteps, w.counter, n);
+      HTR_LAUNCH("chem_update_fp32ir", (chem_update_kernel<MX, HTR_CHEM_FP32_IR>), dim3(blocks), dim3(bs), 0, s, f, mix, g, dt, nsteps, w.counter, n);
    int rc = check_launch("chem_update_kernel");
    if (rc) return rc;
    int fails = 0;
@@ -309,5 +381,6 @@ int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, doubl
 }
 
 template int chem_update_launch<MixAir>(const htr_fields &, const DMix *, const Geo &, double, int, const htr_view &, hipStream_t);
+template int chem_update_launch<Mix30>(const htr_fields &, const DMix *, const Geo &, double, int, const htr_view &, hipStream_t);
 
 } // namespace htr

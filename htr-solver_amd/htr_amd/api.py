@@ -20,7 +20,8 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "update_using_euler_flux", "update_using_hybrid_euler_flux", "update_using_teno_euler_flux",
            "update_using_skew_symmetric_euler_flux", "update_using_diffusion_flux",
            "add_chemistry_sources", "update_chemistry", "calculate_max_spectral_radius",
-           "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report"]
+           "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report",
+           "chem_launch_config", "chem_autotune"]
 
 
 def stream_ptr(stream=None):
@@ -288,3 +289,41 @@ def profile_report():
         name, n, ms = line.rsplit(" ", 2)
         out[name] = (int(n), float(ms))
     return out
+
+
+def chem_launch_config(block, blocks_per_cu):
+    """Launch shape of the persistent implicit-chemistry cell loop."""
+    check(lib().htr_chem_launch_config(int(block), int(blocks_per_cu)))
+
+
+CHEM_CONFIGS = [(64, 4), (64, 8), (64, 16), (128, 4), (128, 8), (256, 2), (256, 4)]
+
+
+def chem_autotune(fs, mix, deltaTime, precision=HTR_CHEM_FP64, grid=None, configs=None, repeats=1):
+    """Times update_chemistry over `grid` for each launch shape (block size,
+    resident blocks per CU) on a copy of the cell states (restored after every
+    trial, so the caller's fields are unchanged), applies the fastest and
+    returns (best, {shape: ms})."""
+    names = ("Conserved", "temperature", "Conserved_t", "Conserved_t_old")
+    saved = {k: fs[k].clone() for k in names}
+    times = {}
+    for cfg in (configs or CHEM_CONFIGS):
+        chem_launch_config(*cfg)
+        best = None
+        for _ in range(repeats):
+            for k in names:
+                fs[k].copy_(saved[k])
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            update_chemistry(fs, deltaTime, mix, precision=precision, grid=grid)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        times[cfg] = best
+    for k in names:
+        fs[k].copy_(saved[k])
+    top = min(times, key=times.get)
+    chem_launch_config(*top)
+    return top, times

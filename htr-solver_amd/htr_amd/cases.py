@@ -12,6 +12,12 @@ meta holds the mixture parameters and scalars of the case.
                      T0 ~ U(2500, 7000) K per seed, p = 1e4 Pa, random-phase
                      solenoidal velocity with E(k) ~ k^-5/3 (1 <= k <= kmax)
                      at Mach_rms 0.4
+  chem30_box(n, seed) cells of the air box (same seed: p, u, and T mapped
+                     linearly from [2500, 7000] K onto [1500, 3000] K, inside
+                     the validity range of the 30-species thermodynamic data)
+                     with 30-species mass fractions ~ Dirichlet(1_30) per cell
+                     (seeded), for the synthetic mechanism of
+                     htr_amd.mechanism.synthetic_30
   wall_metrics(...)  node types and metrics of a direction bounded by
                      staggered walls with tanh stretching
                      (src/prometeo_grid.rg:139-200, src/prometeo_metric.cc:66-100)
@@ -138,6 +144,26 @@ def air_box(n=32, seed=0, p=1.0e4, mach_rms=0.4, kmax=None, T_noise=0.05, sound_
              "MolarFracs": X, "velocity": u / uRef}
     set_metrics(state, [periodic_metrics(n)] * 3, (nx, ny, nz))
     meta = dict(kind="airmix", T0=T0, c0=c0, vorticityScale=1.0, composition=composition, **AIR_REFS)
+    return state, meta
+
+
+def chem30_box(n=16, seed=0, mech_seed=0):
+    from . import mechanism
+    st, meta = air_box(n, seed)
+    mech = mechanism.synthetic_30(mech_seed)
+    rng = np.random.default_rng(seed + 1000)
+    nz = ny = nx = n
+    Y = rng.dirichlet(np.ones(30), size=(nz, ny, nx)).transpose(3, 0, 1, 2).copy()
+    W = np.asarray(mech["W"])[:, None, None, None]
+    MixW = 1.0 / np.sum(Y / W, axis=0)
+    T = st["temperature"] * AIR_REFS["TRef"]
+    Tc = 1500.0 + (T - 2500.0) / 4500.0 * 1500.0
+    state = {"pressure": st["pressure"], "temperature": Tc / mech["TRef"],
+             "MolarFracs": Y * MixW[None] / W, "velocity": st["velocity"]}
+    for k in st:
+        if k.startswith(("nType", "m_")):
+            state[k] = st[k]
+    meta = dict(kind="mechanism", mech=mech, seed=seed, vorticityScale=1.0)
     return state, meta
 
 

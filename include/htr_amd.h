@@ -14,11 +14,13 @@
  *     task's Fluid_bounds argument (periodic wrap is done over it, as
  *     warpPeriodic does), htr_grid.exec is the index space the task runs
  *     over (the domain of its last region argument);
- *   - all FP64 fields of one call share their spatial strides (sx, sy, sz),
- *     which must be non-negative, with every byte offset inside bounds below
- *     2^32 per component; vector fields may use any component stride sc;
- *     the execution rect holds fewer than 2^31 cells (HTR_E_UNSUPPORTED
- *     otherwise);
+ *   - the spatial strides (sx, sy, sz) of every view of one call are an
+ *     integer multiple k >= 1 of one common triple (strides of extent-1
+ *     directions are ignored): SoA fields (k = 1, components sc apart) and the
+ *     reference's per-point arrays VecNEq/VecNSp/Vec3 (k = components, sc = 1)
+ *     both qualify; strides are non-negative, every byte offset inside bounds
+ *     is below 2^32 per component, and the execution rect holds fewer than
+ *     2^31 cells (HTR_E_UNSUPPORTED otherwise);
  *   - all pointers are device pointers; nothing synchronises the stream
  *     except htr_max_spectral_radius, which returns the maximum to the host
  *     (the reference returns a future), and htr_chem_update, which reads its
@@ -179,6 +181,11 @@ int htr_chem_source(const htr_grid *g, const htr_fields *f, const htr_mix *m, vo
  * finishing every cell) when any cell hit the reference's step caps. */
 int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int precision,
                     const htr_mix *m, const htr_diag *diag, void *stream);
+
+/* Launch shape of the persistent cell loop of htr_chem_update (no reference
+ * counterpart: set by the autotuner): threads per block (64, 128, 256) and
+ * resident blocks per CU (1..64).  Applies to later calls of every mixture. */
+int htr_chem_launch_config(int block, int blocks_per_cu);
 
 /* ---- CFL (src/prometeo_cfl.hpp:55-190) ------------------------------------ */
 /* CalculateMaxSpectralRadiusTask::gpu_base_impl (src/prometeo_cfl.cu:97-152);

@@ -12,6 +12,8 @@ TOL = 1e-12          # FP64 parity bound of the hot path (normalised, per compon
 
 
 def oracle_mix(meta):
+    if meta["kind"] == "mechanism":
+        return O.Mix.from_mechanism(meta["mech"])
     if meta["kind"] == "constprop":
         return O.Mix.constprop_mix(meta["R"], meta["gamma"], meta["prandtl"], 0, meta["visc"])
     return O.Mix.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
@@ -19,6 +21,8 @@ def oracle_mix(meta):
 
 def gpu_mix(meta):
     import htr_amd as H
+    if meta["kind"] == "mechanism":
+        return H.Mixture.from_mechanism(meta["mech"])
     if meta["kind"] == "constprop":
         return H.Mixture.constprop(meta["R"], meta["gamma"], meta["prandtl"], 0, meta["visc"])
     return H.Mixture.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
