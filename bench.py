@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
+    ap.add_argument("--chem-n", type=int, default=12, help="edge of the batched-chemistry sample (0: skip)")
+    ap.add_argument("--chem-dt", type=float, default=1e-5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB FP64 tensor 5 times (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -182,6 +184,76 @@ def units_of(fam, name, n, counts):
     return cells
 
 
+def chemistry(n, dt, seed, cpu, threads):
+    """Batched implicit chemistry, 30 species / 180 reversible reactions
+    (34 x 34 Jacobians): solves/s of the FP64 and the FP32-LU + FP64-refinement
+    paths on n^3 cells, launch shapes chosen by the autotuner, FP32+IR against
+    FP64 per species, Rosenbrock steps per cell; CPU baseline: the oracle on
+    a 4^3 sample."""
+    import numpy as np
+    import torch
+    import htr_amd as H
+    from htr_amd import cases
+    st, meta = cases.chem30_box(n, seed)
+    mix = H.Mixture.from_mechanism(meta["mech"])
+    fs = H.FieldSet((n, n, n), 30, nsteps=True)
+    for k, v in st.items():
+        fs.t[k].copy_(torch.from_numpy(np.ascontiguousarray(v)).to(fs.t[k].dtype))
+    H.update_properties_from_primitive(fs, mix)
+    H.update_conserved_from_primitive(fs, mix)
+    fs["Conserved_t"].zero_()
+    init = {k: fs[k].clone() for k in ("Conserved", "temperature")}
+    sub = fs.grid((0, 0, 0), (n - 1, n - 1, max(n // 4, 1) - 1))
+    out = {"workload": "chem30_%d^3" % n, "cells": n ** 3, "dt": dt, "jacobian": "34x34 (nEq)",
+           "mechanism": "CH4_30SpMix species, 180 synthetic reversible reactions (seed 0)"}
+    res = {}
+    for prec, name in ((H.HTR_CHEM_FP64, "fp64"), (H.HTR_CHEM_FP32_IR, "fp32_ir")):
+        best, table = H.chem_autotune(fs, mix, dt, prec, grid=sub)
+        print("chemistry %s: autotune %s -> %s" % (name, {k: round(v, 2) for k, v in table.items()}, best),
+              file=sys.stderr, flush=True)
+        for k in init:
+            fs[k].copy_(init[k])
+        fs["Conserved_t"].zero_()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        H.update_chemistry(fs, dt, mix, precision=prec, diag=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        steps = fs["nsteps"].cpu().numpy().ravel()
+        res[name] = fs["Conserved"][:30].cpu().numpy().copy()
+        print("chemistry %s: %d cells in %.1f ms" % (name, n ** 3, ms), file=sys.stderr, flush=True)
+        out[name] = {"solves_per_s": n ** 3 / (ms * 1e-3), "ms": ms, "launch": {"block": best[0], "blocks_per_cu": best[1]},
+                     "autotune_ms": {"%dx%d" % k: round(v, 3) for k, v in table.items()},
+                     "steps_mean": float(steps.mean()), "steps_p50": float(np.percentile(steps, 50)),
+                     "steps_p99": float(np.percentile(steps, 99)), "steps_max": int(steps.max())}
+        for k in init:
+            fs[k].copy_(init[k])
+    d = np.abs(res["fp32_ir"] - res["fp64"])
+    out["fp32_ir_vs_fp64"] = {"max_abs": float(d.max()),
+                              "max_rel_per_species": [float(x) for x in d.reshape(30, -1).max(axis=1) /
+                                                      np.abs(res["fp64"]).reshape(30, -1).max(axis=1)]}
+    out["speedup_fp32_ir"] = out["fp32_ir"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    if cpu:
+        from oracle import oracle as O
+        m = O.Mix.from_mechanism(meta["mech"])
+        ns = 4
+        g = O.Grid(ns, ns, ns)
+        f = O.Fields(g, 30)
+        sl = (slice(0, ns),) * 3
+        for k in ("Conserved", "temperature"):
+            a = init[k].cpu().numpy()
+            f[k] = a[(slice(None),) + sl] if a.ndim == 4 else a[sl]
+        f["Conserved_t"] = 0.0
+        t0 = time.perf_counter()
+        O.chem_update(g, f, dt, m)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ns ** 3 / el, "unit": "solves/s", "cores": threads, "kind": "port",
+                               "sample": "4^3 cells of the same workload, %.2f s" % el}
+    return out
+
+
 def calibrate_copy():
     """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
     this kernel, divided by its 1 GiB per launch, scale the counters of the hot
@@ -223,6 +295,7 @@ def main():
           file=sys.stderr, flush=True)
     counts = face_counts(H, fs, mix, scheme, dt, n)
     print("TENO faces per direction %s" % counts, file=sys.stderr, flush=True)
+    print("timing %d RHS steps" % (a.warmup + a.steps), file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         run_rhs(H, fs, mix, scheme, dt)
     torch.cuda.synchronize()
@@ -274,8 +347,11 @@ def main():
                          "frac": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "roofline": roof,
     }
+    print("RHS %.3f ms/step" % ms_step, file=sys.stderr, flush=True)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if a.chem_n > 0:
+        result["chemistry"] = chemistry(a.chem_n, a.chem_dt, a.seed, not a.no_cpu, threads)
     if not a.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         rate, reps, el = cpu_baseline(a.cpu_sample, a.seed, scheme, a.cpu_seconds, a.composition)
         result["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                   "sample": "%d^3 box of the same workload, %d full RHS in %.1f s"

@@ -221,8 +221,12 @@ struct Euler {
    }
 
    // ---- FluxReconstruction<Op> (rhs.inl:625-759) in three passes ----------
-   // face record: Roe state and the Lax-Friedrichs eigenvalue maxima
-   static constexpr int NREC = 2*NS + 12;
+   // face record: Roe state, the Lax-Friedrichs eigenvalue maxima, and per
+   // stencil point k the species sums of the left-eigenvector products of the
+   // conserved vector and of the Euler flux (sum_j w_j q_j, sum_j q_j,
+   // sum_j w_j q_j u_n, sum_j q_j u_n with w_j = (Coeff - dpdrhoi_j)/a^2)
+   static constexpr int NREC = 2*NS + 12 + 24;
+   static constexpr int RSUM = 2*NS + 12;        // first sum slot: RSUM + 4k + {0,1,2,3}
 
    // stencil points pM2..pP3 of the face at p (rhs.inl:640-655)
    __device__ __forceinline__ void points(P3 *P, const P3 &p, int t) const
@@ -291,14 +295,11 @@ struct Euler {
       for (int j = 0; j < 3; j++) J.cv[j] = R.v[j]*J.d;
    }
 
-   // ± split flux of component i at point P (getPlusMinusFlux,
-   // rhs.inl:586-620): the species sums of the conserved vector and of the
-   // Euler flux, then row i of the left-eigenvector product of each
-   __device__ __forceinline__ void split(double &vp, double &vm, int i, const P3 &P, double lam, const Roe &R,
-                         const Proj &J, double Ysel) const
+   // species sums of point P for the face's Roe state (the per-point part of
+   // projectToCharacteristicSpace, rhs.inl:443-451)
+   __device__ __forceinline__ void point_sums(const P3 &P, const Roe &R, const Proj &J, double *sums) const
    {
       const double un = vel(P, iN);
-      const double pr = ld<double>(f.pressure, P);
       double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0;
 #pragma unroll
       for (int j = 0; j < NS; j++) {
@@ -310,16 +311,33 @@ struct Euler {
          s1f += w*fq;
          sqf += fq;
       }
+      sums[0] = s1q; sums[1] = sqq; sums[2] = s1f; sums[3] = sqf;
+   }
+
+   // ± split flux of component i at point P (getPlusMinusFlux,
+   // rhs.inl:586-620): row i of the left-eigenvector product of the conserved
+   // vector and of the Euler flux, given the point's species sums
+   __device__ __forceinline__ void split(double &vp, double &vm, int i, const P3 &P, double lam, const Roe &R,
+                         const Proj &J, double Ysel, const double *sums) const
+   {
+      const double un = vel(P, iN);
+      const double pr = ld<double>(f.pressure, P);
       const double qs = (i >= 1 && i <= NS) ? cons(P, i-1) : 0.0;
       double qm[4], fm[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) { qm[j] = cons(P, NS+j); fm[j] = qm[j]*un; }
       fm[iN] += pr;
       fm[3] += pr*un;
-      const double Q = char_row(i, R, J, s1q, sqq, qs, qm, Ysel);
-      const double F = char_row(i, R, J, s1f, sqf, qs*un, fm, Ysel);
+      const double Q = char_row(i, R, J, sums[0], sums[1], qs, qm, Ysel);
+      const double F = char_row(i, R, J, sums[2], sums[3], qs*un, fm, Ysel);
       vp = F + lam*Q;
       vm = F - lam*Q;
+   }
+
+   __device__ static void get_sums(const double *__restrict__ rec, int64_t cap, int64_t w, int k, double *sums)
+   {
+#pragma unroll
+      for (int c = 0; c < 4; c++) sums[c] = rec[(RSUM + 4*k + c)*cap + w];
    }
 
    // pass A: Roe state and eigenvalue maxima of face p
@@ -332,6 +350,16 @@ struct Euler {
       Roe R;
       roe(R, p, P[3]);
       put_rec(rec, cap, w, R, Lam1, Lam, LamN);
+      Proj J;
+      proj(J, R);
+#pragma unroll 1
+      for (int k = 0; k < 6; k++) {
+         double sums[4];
+         const P3 Pk = (k == 2) ? p : (k < 2) ? wminus(g, DIR, p, c_st.Cp[t][k]) : wplus(g, DIR, p, c_st.Cp[t][k-1]);
+         point_sums(Pk, R, J, sums);
+#pragma unroll
+         for (int c = 0; c < 4; c++) rec[(RSUM + 4*k + c)*cap + w] = sums[c];
+      }
    }
 
    // pass B: component i of the face, F_i = -(F+ + F-)/2 in characteristic
@@ -350,7 +378,11 @@ struct Euler {
       const double Ysel = (i >= 1 && i <= NS) ? rec[(i-1)*cap + w] : 0.0;
       double vp[6], vm[6];
 #pragma unroll
-      for (int k = 0; k < 6; k++) split(vp[k], vm[k], i, P[k], lam, R, J, Ysel);
+      for (int k = 0; k < 6; k++) {
+         double sums[4];
+         get_sums(rec, cap, w, k, sums);
+         split(vp[k], vm[k], i, P[k], lam, R, J, Ysel, sums);
+      }
       uint32_t bp, bm;
       const double FPlus  = teno_reconstruct<OP, 0>(vp, t, &bp);
       const double FMinus = teno_reconstruct<OP, 1>(vm, t, &bm);
@@ -399,9 +431,11 @@ struct Euler {
             for (int i = 0; i < NEQ; i++) {
                const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
                const double Ysel = (i >= 1 && i <= NS) ? R.Y[i-1] : 0.0;
-               double ap, am, bp, bm;
-               split(ap, am, i, p, lam, R, J, Ysel);
-               split(bp, bm, i, pP1, lam, R, J, Ysel);
+               double ap, am, bp, bm, s2[4], s3[4];
+               get_sums(rec, cap, w, 2, s2);
+               get_sums(rec, cap, w, 3, s3);
+               split(ap, am, i, p, lam, R, J, Ysel, s2);
+               split(bp, bm, i, pP1, lam, R, J, Ysel, s3);
                add_column(FLO, sq, i, -0.5*(ap + bm), R, ERoe, dedp);
             }
             const double eps = 1e-60;

@@ -167,6 +167,57 @@ def chem30_box(n=16, seed=0, mech_seed=0):
     return state, meta
 
 
+# ---- serving cases ----------------------------------------------------------------
+def shock_tube(n, pressure_ratio, species=1):
+    """1-D shock tube along x: n cells on [0, 1], diaphragm at 0.5, gas at rest,
+    left pressure = pressure_ratio x right, equal temperatures; staggered
+    Dirichlet ghosts at both ends (they keep the initial states).  species 1:
+    calorically perfect gas (gamma 1.4, R 1, p_R = 1, T = 1); 5: air (p_R = 1e4
+    Pa, T = 3000 K, X(N2, O2) = (0.79, 0.21)).  Returns (state, meta) with
+    arrays shaped (…, 1, 1, n + 2)."""
+    nx = n + 2
+    xc = np.concatenate([[0.0], (np.arange(n) + 0.5) / n, [1.0]])
+    left = xc < 0.5
+    if species == 1:
+        pR, T = 1.0, 1.0
+        X = np.ones((1, 1, 1, nx))
+        meta = dict(kind="constprop", R=1.0, gamma=1.4, prandtl=0.71, visc=1.0e-4, vorticityScale=1.0)
+    else:
+        pR, T = 1.0e4 / AIR_REFS["PRef"], 3000.0 / AIR_REFS["TRef"]
+        X = np.zeros((5, 1, 1, nx))
+        X[0], X[1] = 0.79, 0.21
+        X = np.maximum(X, 1e-60)
+        meta = dict(kind="airmix", vorticityScale=1.0, **AIR_REFS)
+    p = np.where(left, pressure_ratio * pR, pR)[None, None, :]
+    state = {"pressure": p.copy(), "temperature": np.full((1, 1, nx), T), "MolarFracs": X,
+             "velocity": np.zeros((3, 1, 1, nx))}
+    nt, me, md, ms, _ = wall_metrics(n, width=1.0, stretching=1e-3)   # uniform to 1e-7
+    one = (np.zeros(1, np.int32), np.ones(1), np.ones(1), np.ones(1))
+    set_metrics(state, [(nt, me, md, ms), one, one], (nx, 1, 1))
+    return state, meta
+
+
+def isentropic_vortex(n, width=10.0, beta=5.0, gamma=1.4):
+    """2-D isentropic vortex advected by u = 1 on a periodic n x n square of
+    side `width` (calorically perfect gas, R = 1; the reference's
+    VortexAdvection2D test family).  Arrays shaped (…, 1, n, n)."""
+    x = cell_centres(n, width) - 0.5 * width
+    X, Y = np.meshgrid(x, x, indexing="xy")               # (y, x)
+    r2 = X ** 2 + Y ** 2
+    f = np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - beta / (2 * math.pi) * Y * f
+    v = beta / (2 * math.pi) * X * f
+    T = 1.0 - (gamma - 1.0) * beta ** 2 / (8 * gamma * math.pi ** 2) * f ** 2
+    rho = T ** (1.0 / (gamma - 1.0))
+    state = {"pressure": (rho * T)[None], "temperature": T[None], "MolarFracs": np.ones((1, 1, n, n)),
+             "velocity": np.stack([u, v, np.zeros_like(u)])[:, None]}
+    px = periodic_metrics(n, width)
+    one = (np.zeros(1, np.int32), np.ones(1), np.ones(1), np.ones(1))
+    set_metrics(state, [px, px, one], (n, n, 1))
+    meta = dict(kind="constprop", R=1.0, gamma=gamma, prandtl=0.71, visc=1.0e-4, vorticityScale=1.0)
+    return state, meta
+
+
 # ---- wall-bounded direction -----------------------------------------------------
 STD, L_S, LP1_S, LP2_S, RM3_S, RM2_S, RM1_S, R_S = range(8)
 
