@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
     ap.add_argument("--chem-n", type=int, default=12, help="edge of the batched-chemistry sample (0: skip)")
     ap.add_argument("--chem-dt", type=float, default=1e-5)
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB FP64 tensor 5 times (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -254,6 +256,25 @@ def chemistry(n, dt, seed, cpu, threads):
     return out
 
 
+def serving(nreq, rate, seed):
+    """Continuous-batching request loop: a seeded Poisson stream of shock tubes
+    (N ~ U{256..16384}, pressure ratio U[2, 20], 1 or 5 species) and 2-D
+    periodic vortices (64^2 .. 256^2), 20-60 SSP-RK3 steps each, admitted into
+    packs by free capacity and HBM budget; p50/p99 of the per-step latency."""
+    from htr_amd import serving as S
+    reqs = S.poisson_requests(nreq, rate, seed=seed, steps=(20, 60), vortex_sizes=(64, 128, 256))
+    srv = S.Server(budget_bytes=32 << 30, tube_capacity=1 << 16, vortex_columns=1024)
+    stats = srv.run(reqs)
+    stats.update({"arrival_rate_per_s": rate, "seed": seed,
+                  "mix": {k: sum(1 for r in reqs if r.kind == k) for k in S.KINDS},
+                  "unit": "cell-steps/s (one SSP-RK3 step of one cell)"})
+    waits = [r.admitted - r.arrival for r in reqs]
+    stats["admission_wait_ms_p50"] = float(sorted(waits)[len(waits) // 2] * 1e3)
+    print("serving: %s" % {k: stats[k] for k in ("rounds", "wall_s", "step_latency_ms_p50", "step_latency_ms_p99")},
+          file=sys.stderr, flush=True)
+    return stats
+
+
 def calibrate_copy():
     """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
     this kernel, divided by its 1 GiB per launch, scale the counters of the hot
@@ -351,6 +372,8 @@ def main():
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if a.chem_n > 0:
         result["chemistry"] = chemistry(a.chem_n, a.chem_dt, a.seed, not a.no_cpu, threads)
+    if a.serve_requests > 0:
+        result["serving"] = serving(a.serve_requests, a.serve_rate, a.seed)
     if not a.no_cpu:
         rate, reps, el = cpu_baseline(a.cpu_sample, a.seed, scheme, a.cpu_seconds, a.composition)
         result["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",

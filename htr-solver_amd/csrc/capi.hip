@@ -61,6 +61,12 @@ int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno
    return rc;
 }
 
+int workspace_enthalpy(Workspace &w, int64_t ndoubles)
+{
+   std::lock_guard<std::mutex> lk(g_mu);
+   return grow((void**)&w.hbuf, w.hbuf_cap, ndoubles, sizeof(double), "enthalpy buffer");
+}
+
 Workspace &workspace()
 {
    int dev = 0;

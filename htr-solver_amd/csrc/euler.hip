@@ -31,6 +31,7 @@ struct Euler {
    const htr_fields &f;
    const DMix &m;
    const Geo &g;
+   const htr_view hv;     // total enthalpy (rhoE + p)/rho per cell, from enthalpy_kernel
 
    __device__ double cons(const P3 &p, int i) const { return ld<double>(f.Conserved, p, i); }
    __device__ double vel(const P3 &p, int i) const { return ld<double>(f.velocity, p, i); }
@@ -40,14 +41,14 @@ struct Euler {
    {
       const double rho_a = ld<double>(f.rho, a), rho_b = ld<double>(f.rho, b);
       const double pa = ld<double>(f.pressure, a), pb = ld<double>(f.pressure, b);
-      const double H = (cons(a, NS+3) + pa)/rho_a;
+      const double H = ld<double>(hv, a);
       const double rhom = rho_a + rho_b;
       const double vm = -(vel(a, iN) + vel(b, iN));
 #pragma unroll
       for (int i = 0; i < NS; i++) S[i] = rhom*vm*(ld<double>(f.MassFracs, a, i) + ld<double>(f.MassFracs, b, i));
 #pragma unroll
       for (int i = 0; i < 3; i++) S[NS+i] = rhom*vm*(vel(a, i) + vel(b, i));
-      S[NS+3] = rhom*vm*(H + (cons(b, NS+3) + pb)/rho_b);
+      S[NS+3] = rhom*vm*(H + ld<double>(hv, b));
       S[NEQ] = (pa + pb);
    }
 
@@ -106,8 +107,8 @@ struct Euler {
       const double pL = ld<double>(f.pressure, L), pR = ld<double>(f.pressure, Rp);
       const double IeL = cons(L, NS+3)/rhoL - 0.5*mod2(vL);
       const double IeR = cons(Rp, NS+3)/rhoR - 0.5*mod2(vR);
-      const double HL = (cons(L, NS+3) + pL)/rhoL;
-      const double HR = (cons(Rp, NS+3) + pR)/rhoR;
+      const double HL = ld<double>(hv, L);
+      const double HR = ld<double>(hv, Rp);
       const double fL = sqrt(rhoL)/(sqrt(rhoL) + sqrt(rhoR));
       const double fR = sqrt(rhoR)/(sqrt(rhoL) + sqrt(rhoR));
       R.rho = sqrt(rhoL*rhoR);
@@ -477,6 +478,24 @@ struct Euler {
 
 };
 
+// Total enthalpy H = (rhoE + p)/rho of every cell of Fluid_bounds (the pair
+// sums and the Roe average read it at stencil points), in a workspace array
+// addressed like the FP64 fields; the same expression the reference evaluates
+// inline (rhs.inl:55, :226-227).
+template<class MX>
+__global__ void __launch_bounds__(256)
+enthalpy_kernel(const htr_fields f, const Geo g, htr_view hv, int64_t ncells)
+{
+   const int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+   if (i >= ncells) return;
+   uint32_t r = (uint32_t)i;
+   const int x = (int)(r % (uint32_t)g.bsz[0]); r /= (uint32_t)g.bsz[0];
+   const int y = (int)(r % (uint32_t)g.bsz[1]);
+   const int z = (int)(r / (uint32_t)g.bsz[1]);
+   const P3 p = mkp(g, x, y, z);
+   at<double>(hv, p) = (ld<double>(f.Conserved, p, MX::NS+3) + ld<double>(f.pressure, p))/ld<double>(f.rho, p);
+}
+
 // ---- face enumeration --------------------------------------------------------
 // A sweep along DIR over the execution rect has (esz[DIR] + 1) faces per line:
 // face j = 0 is the minus face of the first cell, taken at the cell
@@ -527,7 +546,7 @@ __device__ inline void store_face(double *__restrict__ F, int64_t nfaces, uint32
 // face is a KG face (UpdateUsingSkewSymmetricEulerFluxTask).
 template<class MX, int DIR, bool ALL_KG>
 __global__ void __launch_bounds__(256)
-euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ F,
+euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, double *__restrict__ F,
                 int64_t nfaces, int32_t *__restrict__ list, int *__restrict__ counter, htr_view dec)
 {
    const int64_t fid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
@@ -536,7 +555,7 @@ euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, d
    if (active) {
       const P3 q = Faces<DIR>::cell(g, f, (uint32_t)fid);
       const int t = ld<int>(f.nType[DIR], q);
-      Euler<MX, DIR> E{f, *mix, g};
+      Euler<MX, DIR> E{f, *mix, g, hv};
       if (ALL_KG || E.sensor_kg(q, t)) {
          double Flux[MX::NEQ];
          E.kg_flux(Flux, q, t);
@@ -589,12 +608,12 @@ inline TenoBufs teno_bufs(double *base, int64_t cap)
 
 template<class MX, int DIR, bool LISTED>
 __global__ void __launch_bounds__(256)
-euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, TenoBufs b, int64_t nfaces,
+euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b, int64_t nfaces,
                  const int32_t *__restrict__ list, const int *__restrict__ counter)
 {
    const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
    const int64_t stride = (int64_t)gridDim.x*blockDim.x;
-   Euler<MX, DIR> E{f, *mix, g};
+   Euler<MX, DIR> E{f, *mix, g, hv};
    for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < n; w += stride) {
       const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
       const P3 q = Faces<DIR>::cell(g, f, fid);
@@ -604,17 +623,24 @@ euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, 
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
 __global__ void __launch_bounds__(256)
-euler_char_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, TenoBufs b, int64_t nfaces,
+euler_char_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b, int64_t nfaces,
                   const int32_t *__restrict__ list, const int *__restrict__ counter)
 {
    constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
+   constexpr uint32_t CH = 64*MX::NEQ;          // items of one chunk: 64 faces x nEq components
    const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
-   const uint32_t nitems = n*(uint32_t)MX::NEQ;
+   const uint32_t nitems = ((n + 63u)/64u)*CH;
    const uint32_t stride = gridDim.x*blockDim.x;
-   Euler<MX, DIR> E{f, *mix, g};
+   Euler<MX, DIR> E{f, *mix, g, hv};
+   // a wave takes one component of 64 consecutive faces, and the nEq waves of
+   // a chunk are adjacent items, so a face's stencil data and record are read
+   // from HBM once and served from cache to its other components
    for (uint32_t it = blockIdx.x*blockDim.x + threadIdx.x; it < nitems; it += stride) {
-      const uint32_t i = it/n;
-      const uint32_t w = it - i*n;
+      const uint32_t chunk = it/CH;
+      const uint32_t r = it - chunk*CH;
+      const uint32_t i = r >> 6;
+      const uint32_t w = chunk*64u + (r & 63u);
+      if (w >= n) continue;
       const uint32_t fid = LISTED ? (uint32_t)list[w] : w;
       const P3 q = Faces<DIR>::cell(g, f, fid);
       uint32_t bits;
@@ -626,13 +652,14 @@ euler_char_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
 
 template<class MX, int DIR, bool LISTED>
 __global__ void __launch_bounds__(256)
-euler_finish_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double RK0, double RK1,
+euler_finish_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv,
+                    double RK0, double RK1,
                     double dt, TenoBufs b, double *__restrict__ F, int64_t nfaces,
                     const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec)
 {
    const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
    const int64_t stride = (int64_t)gridDim.x*blockDim.x;
-   Euler<MX, DIR> E{f, *mix, g};
+   Euler<MX, DIR> E{f, *mix, g, hv};
    for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < n; w += stride) {
       const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
       const P3 q = Faces<DIR>::cell(g, f, fid);
@@ -666,21 +693,21 @@ face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict
 }
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
-static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1, double dt,
-                       Workspace &w, int64_t nfaces, const htr_view &dec, hipStream_t s)
+static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const htr_view &hv, double RK0,
+                       double RK1, double dt, Workspace &w, int64_t nfaces, const htr_view &dec, hipStream_t s)
 {
    constexpr int kMaxBlocks = 256*16;
    const TenoBufs b = teno_bufs<MX>(w.teno, nfaces);
    const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxBlocks);
-   const unsigned nbc = (unsigned)std::min<int64_t>(grid_for(nfaces*MX::NEQ, 256), kMaxBlocks);
+   const unsigned nbc = (unsigned)std::min<int64_t>(grid_for((nfaces + 63)/64*64*MX::NEQ, 256), kMaxBlocks);
    HTR_LAUNCH(HTR_DIR_NAME("euler_roe", DIR), (euler_roe_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
-                      f, mix, g, b, nfaces, w.list, w.counter);
+                      f, mix, g, hv, b, nfaces, w.list, w.counter);
    int rc = check_launch("euler_roe_kernel"); if (rc) return rc;
    HTR_LAUNCH(HTR_DIR_NAME("euler_char", DIR), (euler_char_kernel<MX, DIR, SCHEME, LISTED>), dim3(nbc), dim3(256), 0, s,
-                      f, mix, g, b, nfaces, w.list, w.counter);
+                      f, mix, g, hv, b, nfaces, w.list, w.counter);
    rc = check_launch("euler_char_kernel"); if (rc) return rc;
    HTR_LAUNCH(HTR_DIR_NAME("euler_finish", DIR), (euler_finish_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
-                      f, mix, g, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec);
+                      f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec);
    return check_launch("euler_finish_kernel");
 }
 
@@ -696,18 +723,28 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    int rc = workspace_faces(w, nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0,
                             SCHEME == HTR_EULER_KG ? 0 : teno_doubles<MX>(nfaces));
    if (rc) return rc;
+   // total enthalpy of every cell of the bounds
+   const int64_t nb_cells = (int64_t)g.bsz[0]*g.bsz[1]*g.bsz[2];
+   const int64_t span = ((int64_t)g.bsz[0] - 1)*g.cb[0] + ((int64_t)g.bsz[1] - 1)*g.cb[1] +
+                        ((int64_t)g.bsz[2] - 1)*g.cb[2] + 1;
+   if ((rc = workspace_enthalpy(w, span))) return rc;
+   htr_view hv = {};
+   hv.ptr = w.hbuf;
+   hv.sx = 8;                                  // one double per base element (k = 1)
+   HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
+   if ((rc = check_launch("enthalpy_kernel"))) return rc;
    if constexpr (SCHEME == HTR_EULER_KG) {
       HTR_LAUNCH(HTR_DIR_NAME("euler_kg", DIR), (euler_kg_kernel<MX, DIR, true>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
-                         f, mix, g, w.faces, nfaces, w.list, w.counter + 1, dec);
+                         f, mix, g, hv, w.faces, nfaces, w.list, w.counter + 1, dec);
       if ((rc = check_launch("euler_kg_kernel"))) return rc;
    } else if constexpr (SCHEME == HTR_EULER_HYBRID) {
       if ((rc = zero_counters(w.counter, s))) return rc;
       HTR_LAUNCH(HTR_DIR_NAME("euler_kg", DIR), (euler_kg_kernel<MX, DIR, false>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
-                         f, mix, g, w.faces, nfaces, w.list, w.counter + 1, dec);
+                         f, mix, g, hv, w.faces, nfaces, w.list, w.counter + 1, dec);
       if ((rc = check_launch("euler_kg_kernel"))) return rc;
-      if ((rc = launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
+      if ((rc = launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
    } else {
-      if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
+      if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
    }
    HTR_LAUNCH(HTR_DIR_NAME("euler_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)), dim3(256), 0, s,
                       f, g, w.faces, nfaces, f.m_e[DIR], ncells);

@@ -21,13 +21,15 @@ struct Workspace {
    double  *faces;         // face fluxes of one sweep, capacity faces_cap doubles
    int32_t *list;          // compacted face indices, capacity list_cap
    double  *teno;          // TENO face records, components and bits, capacity teno_cap doubles
-   int64_t  faces_cap, list_cap, teno_cap;
+   double  *hbuf;          // per-cell total enthalpy of the Euler sweeps, capacity hbuf_cap doubles
+   int64_t  faces_cap, list_cap, teno_cap, hbuf_cap;
    static constexpr int64_t kPartial = 1 << 16;
 };
 Workspace &workspace();
 // grows the face buffers (grow-only; an allocation happens only when a call
 // needs more than any earlier call on this device)
 int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno);
+int workspace_enthalpy(Workspace &w, int64_t ndoubles);
 
 template<class MX> int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
                                       double RK0, double RK1, double dt, const htr_view &dec, hipStream_t s);
