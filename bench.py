@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
+    ap.add_argument("--workload", default="air", choices=["air", "bl"],
+                    help="air: 256^3 periodic air box; bl: 512x256x128 Mach 6 boundary layer")
     ap.add_argument("--chem-n", type=int, default=12, help="edge of the batched-chemistry sample (0: skip)")
     ap.add_argument("--chem-dt", type=float, default=1e-5)
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run (0: skip)")
@@ -78,6 +80,33 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB FP64 tensor 5 times (reference kernel for HBM counters)")
     return ap.parse_args()
+
+
+def setup_bl(seed):
+    """Boundary-layer workload: 512 x 256 x 128 interior cells (y between two
+    staggered wall ghosts), Mach 6 air."""
+    import numpy as np
+    import torch
+    import htr_amd as H
+    from htr_amd import cases
+    st, meta = cases.boundary_layer(512, 256, 128, seed)
+    mix = H.Mixture.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
+    n = (512, 258, 128)
+    fs = H.FieldSet(n, 5)
+    for k, v in st.items():
+        fs.t[k].copy_(torch.from_numpy(np.ascontiguousarray(v)).to(fs.t[k].dtype))
+    del st
+    H.update_properties_from_primitive(fs, mix)
+    H.update_conserved_from_primitive(fs, mix)
+    H.rk_init(fs, 9)
+    H.update_ducros_sensor(fs, meta["vorticityScale"])
+    for d in range(3):
+        H.update_shock_sensor(fs, d, mix)
+    grid = fs.grid((0, 1, 0), (511, 256, 127))
+    r = H.calculate_max_spectral_radius(fs, mix, grid=grid)
+    torch.cuda.synchronize()
+    meta["T0"] = 220.0
+    return fs, mix, 0.5 / r, meta, grid
 
 
 def setup_gpu(n, seed, composition="per-seed"):
@@ -103,12 +132,12 @@ def setup_gpu(n, seed, composition="per-seed"):
     return fs, mix, dt, meta
 
 
-def run_rhs(H, fs, mix, scheme, dt):
+def run_rhs(H, fs, mix, scheme, dt, grid=None):
     """One full RHS, launched task by task in the reference order (zero,
     chemistry source, Euler Z Y X, diffusion Z Y X) on torch's current stream."""
     import ctypes as C
     lib = H.lib()
-    g = C.byref(fs.grid())
+    g = C.byref(grid if grid is not None else fs.grid())
     f = C.byref(fs.c())
     s = H.stream_ptr()
     fs["Conserved_t"].zero_()
@@ -119,18 +148,18 @@ def run_rhs(H, fs, mix, scheme, dt):
         H.check(lib.htr_diffusion_flux(d, g, f, mix.h, s))
 
 
-def face_counts(H, fs, mix, scheme, dt, n):
+def face_counts(H, fs, mix, scheme, dt, n, grid=None):
     """TENO and limiter face counts per direction (one instrumented pass,
     outside the timed region)."""
     import torch
     import numpy as np
-    fs2 = H.FieldSet((n, n, n), 5, decisions=True)
+    fs2 = H.FieldSet(fs.n, 5, decisions=True)
     for k, v in fs.t.items():
         if k in fs2.t:
             fs2.t[k].copy_(v)
     out = {}
     for d in range(3):
-        H.update_using_euler_flux(fs2, d, scheme, (1.0, 0.0), dt, mix, diag=True)
+        H.update_using_euler_flux(fs2, d, scheme, (1.0, 0.0), dt, mix, grid=grid, diag=True)
         dec = fs2["decisions%d" % d].cpu().numpy().astype(np.uint32)
         out[d] = (int(np.sum((dec & 1) == 0)), int(np.sum((dec & 2) != 0)))
     del fs2
@@ -172,13 +201,12 @@ def kernel_family(name):
     return None
 
 
-def units_of(fam, name, n, counts):
+def units_of(fam, name, nfaces_dir, cells, counts):
     """Units one launch of kernel `name` processes."""
-    cells = n ** 3
     d = {"x": 0, "y": 1, "z": 2}.get(name[-1], 0)
     teno = counts[d][0] if counts else 0
     if fam == "euler_kg":
-        return (n + 1) * n * n
+        return nfaces_dir[d]
     if fam in ("euler_roe", "euler_finish"):
         return teno
     if fam == "euler_char":
@@ -309,16 +337,22 @@ def main():
     H.init(0)
     scheme = {"hybrid": H.HTR_EULER_HYBRID, "TENO": H.HTR_EULER_TENO, "KG": H.HTR_EULER_KG}[a.scheme]
     n = a.n
-    cells = n ** 3
     t_setup = time.perf_counter()
-    fs, mix, dt, meta = setup_gpu(n, a.seed, a.composition)
-    print("setup %.1fs, n=%d, dt=%.3e, T0=%.0f K" % (time.perf_counter() - t_setup, n, dt, meta["T0"]),
-          file=sys.stderr, flush=True)
-    counts = face_counts(H, fs, mix, scheme, dt, n)
+    if a.workload == "bl":
+        fs, mix, dt, meta, grid = setup_bl(a.seed)
+        cells = 512 * 256 * 128
+        nfaces_dir = [(513 * 256 * 128), (512 * 257 * 128), (512 * 256 * 129)]
+    else:
+        fs, mix, dt, meta = setup_gpu(n, a.seed, a.composition)
+        grid = None
+        cells = n ** 3
+        nfaces_dir = [(n + 1) * n * n] * 3
+    print("setup %.1fs, %s, dt=%.3e" % (time.perf_counter() - t_setup, a.workload, dt), file=sys.stderr, flush=True)
+    counts = face_counts(H, fs, mix, scheme, dt, n, grid)
     print("TENO faces per direction %s" % counts, file=sys.stderr, flush=True)
     print("timing %d RHS steps" % (a.warmup + a.steps), file=sys.stderr, flush=True)
     for _ in range(a.warmup):
-        run_rhs(H, fs, mix, scheme, dt)
+        run_rhs(H, fs, mix, scheme, dt, grid)
     torch.cuda.synchronize()
     H.profile(True)
     H.profile_report()
@@ -327,7 +361,7 @@ def main():
     torch.cuda.synchronize()
     t0.record()
     for _ in range(a.steps):
-        run_rhs(H, fs, mix, scheme, dt)
+        run_rhs(H, fs, mix, scheme, dt, grid)
     t1.record()
     torch.cuda.synchronize()
     rep = H.profile_report()
@@ -338,7 +372,7 @@ def main():
     per_step = {k: ms / a.steps for k, (cnt, ms) in rep.items()}
     dom = max(per_step, key=per_step.get)
     fam = kernel_family(dom)
-    units = units_of(fam, dom, n, counts) if fam else cells
+    units = units_of(fam, dom, nfaces_dir, cells, counts) if fam else cells
     launch_ms = per_launch[dom]
     bpu = KERNEL_BYTES.get(fam, 0.0)
     achieved = bpu * units / (launch_ms * 1e-3) / 1e9
@@ -349,16 +383,18 @@ def main():
         tf = KERNEL_FLOPS[fam] * units / (launch_ms * 1e-3) / 1e12
         roof["fp64_tflops"] = tf
         roof["fp64_frac"] = tf / FP64_PEAK_TFS
-    faces = 3 * (n + 1) * n * n
+    faces = sum(nfaces_dir)
     result = {
-        "metric": "RHS cell-updates/s (256^3 5-species air, full FP64 RHS)",
+        "metric": "RHS cell-updates/s (%s, 5-species air, full FP64 RHS)" % ("256^3 box" if a.workload == "air" else "512x256x128 boundary layer"),
         "value": cells / (ms_step * 1e-3),
         "unit": "cell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "air5_box_%d^3_rhs_%s_%s" % (n, a.scheme, a.composition), "grid": [n, n, n],
-                   "species": 5, "scheme": a.scheme, "composition": a.composition,
+        "config": {"workload": ("air5_box_%d^3_rhs_%s_%s" % (n, a.scheme, a.composition)) if a.workload == "air"
+                   else "air5_bl_512x256x128_rhs_%s" % a.scheme,
+                   "grid": [n, n, n] if a.workload == "air" else [512, 256, 128],
+                   "species": 5, "scheme": a.scheme, "composition": a.composition if a.workload == "air" else None,
                    "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
         "teno_face_frac": sum(c[0] for c in counts.values()) / faces,

@@ -12,6 +12,13 @@ meta holds the mixture parameters and scalars of the case.
                      T0 ~ U(2500, 7000) K per seed, p = 1e4 Pa, random-phase
                      solenoidal velocity with E(k) ~ k^-5/3 (1 <= k <= kmax)
                      at Mach_rms 0.4
+  boundary_layer(nx, ny, nz, seed)
+                     flat-plate boundary layer, Mach 6 air at 220 K over an
+                     isothermal 300 K wall: x periodic (nx), y wall-normal with
+                     staggered wall ghosts and tanh stretching (max adjacent
+                     spacing ratio 1.04), z periodic; laminar profile from an
+                     error-function velocity and the Crocco-Busemann
+                     temperature, plus 1 % seeded white noise on the velocity
   chem30_box(n, seed) cells of the air box (same seed: p, u, and T mapped
                      linearly from [2500, 7000] K onto [1500, 3000] K, inside
                      the validity range of the 30-species thermodynamic data)
@@ -164,6 +171,56 @@ def chem30_box(n=16, seed=0, mech_seed=0):
         if k.startswith(("nType", "m_")):
             state[k] = st[k]
     meta = dict(kind="mechanism", mech=mech, seed=seed, vorticityScale=1.0)
+    return state, meta
+
+
+def stretching_for_ratio(n, ratio, lo=1e-3, hi=6.0):
+    """tanh stretching parameter whose largest adjacent-spacing ratio of the
+    n interior cells is `ratio` (bisection)."""
+    def worst(s):
+        x = tanh_minus_coords(n, 1.0, s)[1:-1]
+        dx = np.diff(x)
+        r = dx[1:] / dx[:-1]
+        return float(np.max(np.maximum(r, 1.0 / r)))
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if worst(mid) > ratio:
+            hi = mid
+        else:
+            lo = mid
+    return lo
+
+
+def boundary_layer(nx=512, ny=256, nz=128, seed=0, mach=6.0, T_inf=220.0, T_wall=300.0, p=1.0e4,
+                   Lx=0.5, Ly=0.05, Lz=0.1, delta=0.01, noise=0.01):
+    rng = np.random.default_rng(seed)
+    s = stretching_for_ratio(ny, 1.04)
+    nt, me, md, ms, y = wall_metrics(ny, width=Ly, stretching=s)
+    W = 0.79 * AIR_W[0] + 0.21 * AIR_W[1]
+    gamma = 1.4
+    c_inf = math.sqrt(gamma * RGAS / W * T_inf)
+    U = mach * c_inf
+    from math import erf
+    eta = y / delta
+    uu = np.array([erf(2.0 * e) for e in eta])                    # u / U
+    r = 0.85 ** 0.5
+    T_aw = T_inf * (1.0 + r * 0.5 * (gamma - 1.0) * mach ** 2)
+    T = T_wall + (T_aw - T_wall) * uu + (T_inf - T_aw) * uu ** 2   # Crocco-Busemann
+    nyg = ny + 2
+    _, _, uRef = air_refs()
+    vel = np.zeros((3, nz, nyg, nx))
+    vel[0] = (U * uu)[None, :, None]
+    vel[:, :, 1:-1, :] += noise * U * rng.standard_normal((3, nz, ny, nx))
+    X = np.zeros((5, nz, nyg, nx))
+    X[0], X[1] = 0.79, 0.21
+    X = np.maximum(X, 1e-60)
+    state = {"pressure": np.full((nz, nyg, nx), p / AIR_REFS["PRef"]),
+             "temperature": np.broadcast_to((T / AIR_REFS["TRef"])[None, :, None], (nz, nyg, nx)).copy(),
+             "MolarFracs": X, "velocity": vel / uRef}
+    px = periodic_metrics(nx, Lx)
+    pz = periodic_metrics(nz, Lz)
+    set_metrics(state, [px, (nt, me, md, ms), pz], (nx, nyg, nz))
+    meta = dict(kind="airmix", vorticityScale=U / uRef, stretching=s, U=U, **AIR_REFS)
     return state, meta
 
 
