@@ -73,7 +73,8 @@ def parse():
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
     ap.add_argument("--workload", default="air", choices=["air", "bl"],
                     help="air: 256^3 periodic air box; bl: 512x256x128 Mach 6 boundary layer")
-    ap.add_argument("--chem-n", type=int, default=12, help="edge of the batched-chemistry sample (0: skip)")
+    ap.add_argument("--chem-n", type=int, default=0,
+                    help="edge of the batched-chemistry sample (0: skip; the chemistry line is a separate run)")
     ap.add_argument("--chem-dt", type=float, default=1e-5)
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
@@ -292,7 +293,7 @@ def serving(nreq, rate, seed):
     from htr_amd import serving as S
     reqs = S.poisson_requests(nreq, rate, seed=seed, steps=(20, 60), vortex_sizes=(64, 128, 256))
     srv = S.Server(budget_bytes=32 << 30, tube_capacity=1 << 16, vortex_columns=1024)
-    stats = srv.run(reqs)
+    stats = srv.run(reqs, progress=lambda m: print(m, file=sys.stderr, flush=True))
     stats.update({"arrival_rate_per_s": rate, "seed": seed,
                   "mix": {k: sum(1 for r in reqs if r.kind == k) for k in S.KINDS},
                   "unit": "cell-steps/s (one SSP-RK3 step of one cell)"})

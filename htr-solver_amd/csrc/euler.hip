@@ -650,6 +650,125 @@ euler_char_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    }
 }
 
+// Pass 2b with the face data staged in LDS: a block of nEq waves takes a
+// chunk of 64 TENO faces; all its threads load, with coalesced reads, the
+// values the components need (per face: the Roe state, the eigenvalue
+// maxima and the per-point species sums of the record; per stencil point:
+// the conserved vector, u_n and p) into LDS once, then wave i computes
+// component i of the 64 faces from LDS.
+template<class MX>
+struct CharStage {
+   static constexpr int NS = MX::NS, NEQ = MX::NEQ;
+   static constexpr int NPV = NS + 6;                // per point: rhoY_0..NS-1, rhou 3, rhoE, u_n, p
+   static constexpr int NRV = NS + 35;               // per face: Y NS, v 3, a, e, rho, dpde, a2, Lam 3, sums 24
+   double pt[6*NPV][64];
+   double rc[NRV][64];
+   int    t[64];
+   uint32_t fid[64];
+};
+
+template<class MX, int DIR, int SCHEME, bool LISTED>
+__global__ void __launch_bounds__(64*MX::NEQ)
+euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b,
+                      int64_t nfaces, const int32_t *__restrict__ list, const int *__restrict__ counter)
+{
+   using E_t = Euler<MX, DIR>;
+   constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = CharStage<MX>::NPV, NRV = CharStage<MX>::NRV;
+   constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
+   constexpr int iN = E_t::iN;
+   __shared__ CharStage<MX> S;
+   const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
+   const uint32_t nchunks = (n + 63u)/64u;
+   const int lane = threadIdx.x & 63;
+   const int wv = threadIdx.x >> 6;                   // wave = component
+   E_t E{f, *mix, g, hv};
+   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+      const uint32_t w = c*64u + (uint32_t)lane;
+      const bool valid = w < n;
+      // ---- stage: every wave computes its lane's face geometry, then loads
+      // a share of the point values and of the record
+      P3 q = mkp(g, 0, 0, 0);
+      int t = 0;
+      if (valid) {
+         const uint32_t fd = LISTED ? (uint32_t)list[w] : w;
+         q = Faces<DIR>::cell(g, f, fd);
+         t = ld<int>(f.nType[DIR], q);
+         if (wv == 0) { S.t[lane] = t; S.fid[lane] = fd; }
+      }
+      for (int v = wv; v < NPV; v += NEQ) {
+#pragma unroll
+         for (int k = 0; k < 6; k++) {
+            double x = 0.0;
+            if (valid) {
+               const P3 P = (k == 2) ? q : (k < 2) ? wminus(g, DIR, q, c_st.Cp[t][k]) : wplus(g, DIR, q, c_st.Cp[t][k-1]);
+               x = (v < NS + 4) ? ld<double>(f.Conserved, P, v) : (v == NS + 4) ? ld<double>(f.velocity, P, iN)
+                                                                                  : ld<double>(f.pressure, P);
+            }
+            S.pt[k*NPV + v][lane] = x;
+         }
+      }
+      for (int r = wv; r < NRV; r += NEQ) {
+         // record field of slot r: Y 0..NS-1 -> rec 0..NS-1; v, a, H, e, rho, dpde -> NS..NS+7 minus H;
+         int src;
+         if (r < NS) src = r;                                   // Y
+         else if (r < NS + 4) src = r;                          // v[3], a
+         else if (r < NS + 7) src = r + 1;                      // e, rho, dpde (skip H)
+         else if (r < NS + 8) src = 2*NS + 8;                   // a2
+         else if (r < NS + 11) src = 2*NS + 9 + (r - NS - 8);   // Lam1, Lam, LamN
+         else src = E_t::RSUM + (r - NS - 11);                  // sums
+         S.rc[r][lane] = valid ? b.rec[(int64_t)src*b.cap + w] : 0.0;
+      }
+      __syncthreads();
+      // ---- compute: wave wv = component i of the 64 faces
+      if (valid && wv < NEQ) {
+         const int i = wv;
+         typename E_t::Roe R;
+#pragma unroll
+         for (int s = 0; s < NS; s++) { R.Y[s] = S.rc[s][lane]; R.dpdrhoi[s] = 0.0; }
+#pragma unroll
+         for (int j = 0; j < 3; j++) R.v[j] = S.rc[NS + j][lane];
+         R.a = S.rc[NS + 3][lane];
+         R.H = 0.0;
+         R.e = S.rc[NS + 4][lane];
+         R.rho = S.rc[NS + 5][lane];
+         R.dpde = S.rc[NS + 6][lane];
+         R.a2 = S.rc[NS + 7][lane];
+         const double lam = S.rc[NS + 8 + ((i == 0) ? 0 : (i == NEQ-1) ? 2 : 1)][lane];
+         typename E_t::Proj J;
+         E_t::proj(J, R);
+         const double Ysel = (i >= 1 && i <= NS) ? S.rc[i-1][lane] : 0.0;
+         double vp[6], vm[6];
+#pragma unroll
+         for (int k = 0; k < 6; k++) {
+            const double *pv = &S.pt[k*NPV][0];
+            double sums[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) sums[cc] = S.rc[NS + 11 + 4*k + cc][lane];
+            const double un = S.pt[k*NPV + NS + 4][lane];
+            const double pr = S.pt[k*NPV + NS + 5][lane];
+            const double qs = (i >= 1 && i <= NS) ? S.pt[k*NPV + i - 1][lane] : 0.0;
+            double qm[4], fm[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) { qm[j] = S.pt[k*NPV + NS + j][lane]; fm[j] = qm[j]*un; }
+            (void)pv;
+            fm[iN] += pr;
+            fm[3] += pr*un;
+            const double Q = E_t::char_row(i, R, J, sums[0], sums[1], qs, qm, Ysel);
+            const double F = E_t::char_row(i, R, J, sums[2], sums[3], qs*un, fm, Ysel);
+            vp[k] = F + lam*Q;
+            vm[k] = F - lam*Q;
+         }
+         uint32_t bp, bm;
+         const int tt = S.t[lane];
+         const double FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
+         const double FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
+         b.Fc[(int64_t)i*b.cap + w] = -0.5*(FPlus + FMinus);
+         b.bits[(int64_t)i*b.cap + w] = (uint8_t)(bp | (bm << 4));
+      }
+      __syncthreads();
+   }
+}
+
 template<class MX, int DIR, bool LISTED>
 __global__ void __launch_bounds__(256)
 euler_finish_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv,
@@ -703,8 +822,9 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
    HTR_LAUNCH(HTR_DIR_NAME("euler_roe", DIR), (euler_roe_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
                       f, mix, g, hv, b, nfaces, w.list, w.counter);
    int rc = check_launch("euler_roe_kernel"); if (rc) return rc;
-   HTR_LAUNCH(HTR_DIR_NAME("euler_char", DIR), (euler_char_kernel<MX, DIR, SCHEME, LISTED>), dim3(nbc), dim3(256), 0, s,
-                      f, mix, g, hv, b, nfaces, w.list, w.counter);
+   const unsigned nchunk = (unsigned)std::min<int64_t>((nfaces + 63)/64, 256*8);
+   HTR_LAUNCH(HTR_DIR_NAME("euler_char", DIR), (euler_char_lds_kernel<MX, DIR, SCHEME, LISTED>), dim3(nchunk),
+              dim3(64*MX::NEQ), 0, s, f, mix, g, hv, b, nfaces, w.list, w.counter);
    rc = check_launch("euler_char_kernel"); if (rc) return rc;
    HTR_LAUNCH(HTR_DIR_NAME("euler_finish", DIR), (euler_finish_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
                       f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec);

@@ -242,7 +242,7 @@ class Server:
         self.packs.setdefault(key, []).append(p)
         return p if p.try_admit(req) else None
 
-    def run(self, requests, keep_results=False, clock=time.perf_counter):
+    def run(self, requests, keep_results=False, clock=time.perf_counter, progress=None):
         """Serves the requests (sorted by arrival, seconds after start); returns
         statistics.  Per-step latency of a case = wall time of the round that
         advanced it."""
@@ -275,6 +275,9 @@ class Server:
             torch.cuda.synchronize()
             ms = (clock() - ts) * 1e3
             rounds += 1
+            if progress is not None and rounds % 50 == 0:
+                progress("serving: round %d, %.1f s, %d resident, %d waiting"
+                         % (rounds, clock() - t0, sum(len(p.resident) for p in active), len(queue) + len(pending)))
             for p in active:
                 for rid, (req, x0) in list(p.resident.items()):
                     req.done_steps += 1
