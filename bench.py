@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--chem-n", type=int, default=0,
                     help="edge of the batched-chemistry sample (0: skip; the chemistry line is a separate run)")
     ap.add_argument("--chem-dt", type=float, default=1e-5)
+    ap.add_argument("--chem-only", action="store_true",
+                    help="only the batched-chemistry measurement (its own JSON line; --chem-n, --chem-dt)")
+    ap.add_argument("--chem-tune", action="store_true", help="run the launch-shape autotuner first")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -215,7 +218,7 @@ def units_of(fam, name, nfaces_dir, cells, counts):
     return cells
 
 
-def chemistry(n, dt, seed, cpu, threads):
+def chemistry(n, dt, seed, cpu, threads, tune=True):
     """Batched implicit chemistry, 30 species / 180 reversible reactions
     (34 x 34 Jacobians): solves/s of the FP64 and the FP32-LU + FP64-refinement
     paths on n^3 cells, launch shapes chosen by the autotuner, FP32+IR against
@@ -239,9 +242,14 @@ def chemistry(n, dt, seed, cpu, threads):
            "mechanism": "CH4_30SpMix species, 180 synthetic reversible reactions (seed 0)"}
     res = {}
     for prec, name in ((H.HTR_CHEM_FP64, "fp64"), (H.HTR_CHEM_FP32_IR, "fp32_ir")):
-        best, table = H.chem_autotune(fs, mix, dt, prec, grid=sub)
-        print("chemistry %s: autotune %s -> %s" % (name, {k: round(v, 2) for k, v in table.items()}, best),
-              file=sys.stderr, flush=True)
+        if tune:
+            best, table = H.chem_autotune(fs, mix, dt, prec, grid=sub,
+                                          configs=[(64, 4), (64, 16), (128, 8), (256, 4)])
+            print("chemistry %s: autotune %s -> %s" % (name, {k: round(v, 2) for k, v in table.items()}, best),
+                  file=sys.stderr, flush=True)
+        else:
+            best, table = (64, 16), {}
+            H.chem_launch_config(*best)
         for k in init:
             fs[k].copy_(init[k])
         fs["Conserved_t"].zero_()
@@ -328,6 +336,16 @@ def main():
     a = parse()
     if a.calibrate_copy:
         calibrate_copy()
+        return
+    if a.chem_only:
+        import htr_amd as H
+        H.init(0)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out = chemistry(max(a.chem_n, 4), a.chem_dt, a.seed, not a.no_cpu, threads, a.chem_tune)
+        out.update({"metric": "implicit-chemistry solves/s (30 species, 34x34 Jacobians), FP32+IR vs FP64",
+                    "value": out["fp32_ir"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+                    "higher_is_better": True, "dtype": "f64 (FP32 LU + FP64 refinement)", "data": "synthetic"})
+        print(json.dumps(out), flush=True)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's share of the "

@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
+#include <map>
 #include <mutex>
 #include <vector>
 #include "common.hpp"
@@ -37,13 +38,18 @@ int check_launch(const char *what)
 }
 
 static std::mutex g_mu;
-static Workspace g_ws[16];
+static std::map<std::pair<int, uintptr_t>, Workspace*> g_ws;   // (device, stream) -> workspace
 static bool g_ready[16];
 
-static int grow(void **p, int64_t &cap, int64_t need, size_t elem, const char *what)
+// grows a workspace buffer; the old buffer is freed only after the stream's
+// earlier work (the only work that can use it) has finished
+static int grow(hipStream_t s, void **p, int64_t &cap, int64_t need, size_t elem, const char *what)
 {
    if (need <= cap) return HTR_OK;
-   if (*p) (void)hipFree(*p);
+   if (*p) {
+      if (hipStreamSynchronize(s) != hipSuccess) return set_error(HTR_E_DEVICE, "stream synchronisation failed");
+      (void)hipFree(*p);
+   }
    *p = nullptr;
    cap = 0;
    if (hipMalloc(p, elem*(size_t)need) != hipSuccess)
@@ -55,23 +61,37 @@ static int grow(void **p, int64_t &cap, int64_t need, size_t elem, const char *w
 int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno)
 {
    std::lock_guard<std::mutex> lk(g_mu);
-   int rc = grow((void**)&w.faces, w.faces_cap, ndoubles, sizeof(double), "face buffer");
-   if (!rc) rc = grow((void**)&w.list, w.list_cap, nlist, sizeof(int32_t), "face list");
-   if (!rc) rc = grow((void**)&w.teno, w.teno_cap, nteno, sizeof(double), "TENO face buffer");
+   int rc = grow(w.stream, (void**)&w.faces, w.faces_cap, ndoubles, sizeof(double), "face buffer");
+   if (!rc) rc = grow(w.stream, (void**)&w.list, w.list_cap, nlist, sizeof(int32_t), "face list");
+   if (!rc) rc = grow(w.stream, (void**)&w.teno, w.teno_cap, nteno, sizeof(double), "TENO face buffer");
    return rc;
 }
 
 int workspace_enthalpy(Workspace &w, int64_t ndoubles)
 {
    std::lock_guard<std::mutex> lk(g_mu);
-   return grow((void**)&w.hbuf, w.hbuf_cap, ndoubles, sizeof(double), "enthalpy buffer");
+   return grow(w.stream, (void**)&w.hbuf, w.hbuf_cap, ndoubles, sizeof(double), "enthalpy buffer");
 }
 
-Workspace &workspace()
+Workspace *workspace(hipStream_t s)
 {
    int dev = 0;
    (void)hipGetDevice(&dev);
-   return g_ws[dev & 15];
+   std::lock_guard<std::mutex> lk(g_mu);
+   Workspace *&w = g_ws[std::make_pair(dev, (uintptr_t)s)];
+   if (w) return w;
+   Workspace *n = new Workspace();
+   memset(n, 0, sizeof(Workspace));
+   n->stream = s;
+   if (hipMalloc(&n->partial, sizeof(double)*Workspace::kPartial) != hipSuccess ||
+       hipHostMalloc(&n->result_host, sizeof(double)) != hipSuccess ||
+       hipMalloc(&n->counter, sizeof(int)*64) != hipSuccess) {
+      set_error(HTR_E_DEVICE, "workspace allocation failed");
+      delete n;
+      return nullptr;
+   }
+   w = n;
+   return w;
 }
 
 static int ensure_device()
@@ -81,11 +101,6 @@ static int ensure_device()
    std::lock_guard<std::mutex> lk(g_mu);
    if (g_ready[dev & 15]) return HTR_OK;
    upload_stencil_tables();
-   Workspace &w = g_ws[dev & 15];
-   if (hipMalloc(&w.partial, sizeof(double)*Workspace::kPartial) != hipSuccess ||
-       hipHostMalloc(&w.result_host, sizeof(double)) != hipSuccess ||
-       hipMalloc(&w.counter, sizeof(int)*64) != hipSuccess)
-      return set_error(HTR_E_DEVICE, "workspace allocation failed");
    g_ready[dev & 15] = true;
    return HTR_OK;
 }

@@ -362,7 +362,9 @@ int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, doubl
 {
    const int64_t n = exec_cells(g);
    if (n == 0) return HTR_OK;
-   Workspace &w = workspace();
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   Workspace &w = *wp;
    if (int rc0 = zero_counters(w.counter, s)) return rc0;
    const int bs = g_chem_block;
    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, bs), 256*g_chem_blocks_per_cu);

@@ -154,7 +154,9 @@ static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipSt
    if (ncells == 0) return HTR_OK;
    const int64_t nfaces = Faces<DIR>::count(g);
    if (nfaces*MX::NEQ >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many faces");
-   Workspace &w = workspace();
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   Workspace &w = *wp;
    int rc = workspace_faces(w, nfaces*MX::NEQ, 0, 0);
    if (rc) return rc;
    HTR_LAUNCH(HTR_DIR_NAME("diffusion_face", DIR), (diffusion_face_kernel<MX, DIR>), dim3(grid_for(nfaces, 256)),

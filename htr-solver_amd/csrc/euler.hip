@@ -839,7 +839,9 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    if (ncells == 0) return HTR_OK;
    const int64_t nfaces = Faces<DIR>::count(g);
    if (nfaces*MX::NEQ >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many faces");
-   Workspace &w = workspace();
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   Workspace &w = *wp;
    int rc = workspace_faces(w, nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0,
                             SCHEME == HTR_EULER_KG ? 0 : teno_doubles<MX>(nfaces));
    if (rc) return rc;

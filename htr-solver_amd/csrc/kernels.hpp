@@ -13,8 +13,12 @@ using Mix30  = Mix<30, false>;  // 30-species mechanism (implicit chemistry)
 int  set_error(int code, const char *fmt, ...);
 int  check_launch(const char *what);
 
-// per-device workspace owned by the library (allocated once in htr_init)
+// Workspace owned by the library, one per (device, stream): kernels on
+// different streams may run concurrently, so no two streams share scratch
+// buffers or counters.  Made at the first call on a stream; the face buffers
+// grow (only) when a call needs more.
 struct Workspace {
+   hipStream_t stream;
    double  *partial;       // per-block partial maxima, capacity kPartial
    double  *result_host;   // pinned, 1 double
    int     *counter;       // device ints: [0] failure counter, [1] face-list length
@@ -25,7 +29,7 @@ struct Workspace {
    int64_t  faces_cap, list_cap, teno_cap, hbuf_cap;
    static constexpr int64_t kPartial = 1 << 16;
 };
-Workspace &workspace();
+Workspace *workspace(hipStream_t s);   // null on allocation failure (error set)
 // grows the face buffers (grow-only; an allocation happens only when a call
 // needs more than any earlier call on this device)
 int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno);

@@ -328,7 +328,9 @@ template<class MX>
 int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, double *result, hipStream_t s)
 {
    const int64_t n = exec_cells(g);
-   Workspace &w = workspace();
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   Workspace &w = *wp;
    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(grid_for(n, 256), 1), 4096);
    HTR_LAUNCH("spectral_radius", spectral_radius_kernel<MX>, dim3((unsigned)blocks), dim3(256), 0, s, f, mix, g, n, w.partial);
    int rc = check_launch("spectral_radius_kernel");
