@@ -36,8 +36,12 @@ RHS_BYTES_PER_CELL = 495.0
 
 # Per-kernel algorithmic bytes per unit (AirMix, NS = 5, nEq = 9); the unit
 # count of one launch comes from the workload (cells, faces, TENO faces).
-#   euler_kg        per face: own-cell rho, Y 5, u 3, p, rhoE (88 B) + nType
-#                   + sensor (5 B) read, face flux (72 B) written = 165 B
+#   euler_classify  per face: nType + own-cell sensor (5 B) read
+#   euler_kg_species per cell (KG walk, species group): rho, u_n, Y 5 (56 B)
+#                   + nType + sensor (5 B) + m_e (8 B) read, Conserved_t[0:5]
+#                   read-modify-write (80 B) = 149 B
+#   euler_kg_flow   per cell (KG walk, momentum/energy group): rho, u 3, H,
+#                   p (48 B) + 13 B as above, Conserved_t[5:9] RMW (64 B) = 125 B
 #   euler_roe       per TENO face: own-cell Roe inputs (88 B) + SoS (8 B)
 #                   read, face record (22 doubles, 176 B) written = 272 B
 #   euler_char      per (component, TENO face): 1/9 of the record (20 B),
@@ -51,7 +55,7 @@ RHS_BYTES_PER_CELL = 495.0
 #                   Conserved_t read-modify-write 144 B = 356 B
 #   chem_source     per cell: rho, p, T, Y 5 (64 B) + Conserved_t[0:5] RMW
 #                   (80 B) = 144 B
-KERNEL_BYTES = {"euler_kg": 165.0, "euler_roe": 272.0, "euler_char": 29.0, "euler_finish": 329.0,
+KERNEL_BYTES = {"euler_classify": 5.0, "euler_kg_species": 149.0, "euler_kg_flow": 125.0, "euler_roe": 272.0, "euler_char": 29.0, "euler_finish": 329.0,
                 "euler_divergence": 224.0, "diffusion": 356.0, "chem_source_kernel": 144.0}
 # FP64 operations per unit of the compute-bound kernels (hand count of the
 # source: split fluxes of six points 528, two TENO reconstructions ~300, the
@@ -198,6 +202,23 @@ def cpu_baseline(n, seed, scheme, budget_s, composition="per-seed"):
     return reps * n ** 3 / el, reps, el
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "rhs_air5_256_hbm_traffic.md")
+
+
+def measured_traffic(launch):
+    """HBM bytes per launch of `launch` on the default workload, from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_TABLE) as fh:
+            for line in fh:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) == 6 and cols[0] == launch and cols[5].isdigit():
+                    return int(cols[5])
+    except OSError:
+        pass
+    return None
+
+
 def kernel_family(name):
     for k in KERNEL_BYTES:
         if name.startswith(k):
@@ -209,7 +230,7 @@ def units_of(fam, name, nfaces_dir, cells, counts):
     """Units one launch of kernel `name` processes."""
     d = {"x": 0, "y": 1, "z": 2}.get(name[-1], 0)
     teno = counts[d][0] if counts else 0
-    if fam == "euler_kg":
+    if fam == "euler_classify":
         return nfaces_dir[d]
     if fam in ("euler_roe", "euler_finish"):
         return teno
@@ -398,6 +419,12 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_unit": bpu, "units_per_launch": units,
             "launch_ms": launch_ms}
+    default_workload = a.workload == "air" and n == 256 and a.composition == "per-seed" and a.scheme == "hybrid"
+    traffic = measured_traffic(dom) if default_workload else None
+    if traffic is not None:
+        roof["traffic"] = traffic
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_TABLE, ROOT)
+        roof["algorithmic_bytes"] = bpu * units
     if fam in KERNEL_FLOPS:
         tf = KERNEL_FLOPS[fam] * units / (launch_ms * 1e-3) / 1e12
         roof["fp64_tflops"] = tf

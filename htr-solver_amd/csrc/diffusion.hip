@@ -2,9 +2,8 @@
 // velocity) flux divergence along one direction.
 //
 // Reference: src/prometeo_rhs.inl:1058-1282 (GetSigma, GetDiffusionFlux,
-// updateRHSSpan), src/prometeo_rhs.cu:461-622.  Two passes per direction, as
-// for the Euler fluxes: every face flux into the face buffer (one lane per
-// face, x-fastest), then the coalesced divergence with m_d.
+// updateRHSSpan), src/prometeo_rhs.cu:461-622.  One pass per direction:
+// line segments walked face by face, the divergence fused (below).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -36,13 +35,23 @@ struct Diffusion {
       }
    }
 
+   // face flux of cell p with the carried transverse gradients g[0] (left)
+   // and those of q = p + 1 formed here into g[1]; g[1] moves to g[0]
    __device__ void flux(double *Flux, const P3 &p, double g1[2][3], double g2[2][3]) const
    {
-      const int t = ld<int>(f.nType[DIR], p);
-      const double m_s = ld<double>(f.m_s[DIR], p);
       const P3 q = wplus(g, DIR, p, 1);
       deriv3<D1>(q, g1[1]);
       deriv3<D2>(q, g2[1]);
+      flux_at(Flux, p, q, g1, g2);
+#pragma unroll
+      for (int i = 0; i < 3; i++) { g1[0][i] = g1[1][i]; g2[0][i] = g2[1][i]; }
+   }
+
+   // GetDiffusionFlux (rhs.inl:1140-1218) at the face between p and q
+   __device__ void flux_at(double *Flux, const P3 &p, const P3 &q, const double g1[2][3], const double g2[2][3]) const
+   {
+      const int t = ld<int>(f.nType[DIR], p);
+      const double m_s = ld<double>(f.m_s[DIR], p);
       double Xp[NS], Xq[NS];
 #pragma unroll
       for (int i = 0; i < NS; i++) { Xp[i] = ld<double>(f.MolarFracs, p, i); Xq[i] = ld<double>(f.MolarFracs, q, i); }
@@ -104,34 +113,43 @@ struct Diffusion {
 #pragma unroll
       for (int i = 0; i < 3; i++) uSigma += interp2(t, vp[i], vq[i])*s[i];
       Flux[NS+3] = (uSigma - heatFlux);
-#pragma unroll
-      for (int i = 0; i < 3; i++) { g1[0][i] = g1[1][i]; g2[0][i] = g2[1][i]; }
    }
 };
 
-// One lane per face (the face enumeration of the Euler passes): face j = 0
+// Faces walked along the line, divergence fused.  A thread takes one line
+// and one segment of L faces [j0, j0 + L) (a segment other than the first
+// also forms face j0 - 1), in the face order of the Euler passes: face j = 0
 // is the minus face of the line's first cell p0, at pm1 = p0 + offM1(nType);
 // face j >= 1 is the plus face of cell elo + j - 1.  The transverse velocity
-// gradients entering a face's left value are those the reference carries from
-// the previous face's right cell (rhs.inl:1212-1214): the face cell itself,
-// except for j = 1, where they belong to pm1 + 1 (different from p0 only when
-// offM1(nType[p0]) = 0, a staggered wall node).
+// gradients entering a face's left value are carried from the previous
+// face's right cell, as the reference does (rhs.inl:1212-1214): the face
+// cell itself, except for j = 1, where they belong to pm1 + 1 (different
+// from p0 only when offM1(nType[p0]) = 0, a staggered wall node).  Face j's
+// flux is differenced with face j-1's into Conserved_t of the cell between
+// them with m_d (rhs.inl:1264-1279).
 template<class MX, int DIR>
-__global__ void __launch_bounds__(256)
-diffusion_face_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ F,
-                      int64_t nfaces)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int L, uint32_t nseg,
+                      uint32_t nunits)
 {
-   const int64_t fid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
-   if (fid >= nfaces) return;
-   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B;
-   uint32_t line; int j;
-   Faces<DIR>::split(g, (uint32_t)fid, line, j);
+   const uint32_t u = blockIdx.x*blockDim.x + threadIdx.x;
+   if (u >= nunits) return;
+   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B, NEQ = MX::NEQ;
+   uint32_t line, seg;
+   if (DIR == 0) { line = u/nseg; seg = u - line*nseg; }
+   else {
+      const uint32_t nl = (uint32_t)g.esz[A]*(uint32_t)g.esz[B];
+      seg = u/nl; line = u - seg*nl;
+   }
+   const int j0 = (int)seg*L;
+   const int j1 = min(j0 + L, g.esz[DIR] + 1);
    int c[3];
    c[A] = g.elo[A] + (int)(line % (uint32_t)g.esz[A]);
    c[B] = g.elo[B] + (int)(line / (uint32_t)g.esz[A]);
    c[DIR] = g.elo[DIR];
    const P3 p0 = mkp(g, c[0], c[1], c[2]);
    const P3 pm1 = wminus(g, DIR, p0, c_st.Cp[ld<int>(f.nType[DIR], p0)][1]);
+   int j = (j0 == 0) ? 0 : j0 - 1;
    P3 cell = pm1, gl = pm1;
    if (j > 0) {
       cell = p0;
@@ -140,11 +158,75 @@ diffusion_face_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    }
    using D_t = Diffusion<MX, DIR>;
    D_t D{f, *mix, g};
-   double g1[2][3], g2[2][3], Flux[MX::NEQ];
+   double g1[2][3], g2[2][3], Fm[NEQ], Fc[NEQ];
    D.template deriv3<D_t::D1>(gl, g1[0]);
    D.template deriv3<D_t::D2>(gl, g2[0]);
-   D.flux(Flux, cell, g1, g2);
-   store_face<MX::NEQ>(F, nfaces, (uint32_t)fid, Flux);
+   D.flux(Fm, cell, g1, g2);
+   for (j++; j < j1; j++) {
+      cell = (j == 1) ? p0 : wplus(g, DIR, cell, 1);
+      D.flux(Fc, cell, g1, g2);
+      const double m = ld<double>(f.m_d[DIR], cell);
+#pragma unroll
+      for (int i = 0; i < NEQ; i++) {
+         at<double>(f.Conserved_t, cell, i) += (Fc[i] - Fm[i])*m;
+         Fm[i] = Fc[i];
+      }
+   }
+}
+
+// x sweep: lanes on consecutive faces of one row.  A wave takes XD_OUT = 62
+// faces (lanes 2..63): every lane forms the transverse gradients of its
+// right cell q, and takes those of its left cell and its FluxM from lane l-1
+// (wave shuffles), so lanes 0 and 1 serve as the halo.  Face 0 takes its
+// left gradients at pm1 itself (rhs.inl:1253-1254).
+constexpr int XD_OUT = 62;
+
+template<class MX>
+__global__ void __launch_bounds__(256)
+diffusion_x_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, uint32_t nchunk, uint32_t nwaves)
+{
+   constexpr int NEQ = MX::NEQ;
+   using D_t = Diffusion<MX, 0>;
+   const uint32_t wave = (blockIdx.x*blockDim.x + threadIdx.x) >> 6;
+   if (wave >= nwaves) return;                                      // whole waves
+   const int lane = threadIdx.x & 63;
+   const uint32_t row = wave/nchunk, k = wave - row*nchunk;
+   const int esz = g.esz[0];
+   const int j = (int)k*XD_OUT + lane - 2;                           // face of this lane
+   const P3 prow = mkp(g, g.elo[0], g.elo[1] + (int)(row % (uint32_t)g.esz[1]),
+                       g.elo[2] + (int)(row/(uint32_t)g.esz[1]));
+   const bool live = j <= esz;
+   P3 p = wplus(g, 0, prow, j - 1);                                  // left cell of face j
+   if (j == 0) p = wminus(g, 0, prow, c_st.Cp[ld<int>(f.nType[0], prow)][1]);
+   const P3 q = wplus(g, 0, p, 1);
+   D_t D{f, *mix, g};
+   double g1[2][3], g2[2][3];
+#pragma unroll
+   for (int i = 0; i < 3; i++) { g1[1][i] = 0.0; g2[1][i] = 0.0; }
+   if (live) {
+      D.template deriv3<D_t::D1>(q, g1[1]);
+      D.template deriv3<D_t::D2>(q, g2[1]);
+   }
+#pragma unroll
+   for (int i = 0; i < 3; i++) { g1[0][i] = __shfl_up(g1[1][i], 1, 64); g2[0][i] = __shfl_up(g2[1][i], 1, 64); }
+   double Fc[NEQ];
+#pragma unroll
+   for (int i = 0; i < NEQ; i++) Fc[i] = 0.0;
+   if (lane >= 1 && live && j >= 0) {
+      if (j == 0) {
+         D.template deriv3<D_t::D1>(p, g1[0]);
+         D.template deriv3<D_t::D2>(p, g2[0]);
+      }
+      D.flux_at(Fc, p, q, g1, g2);
+   }
+   double Fm[NEQ];
+#pragma unroll
+   for (int i = 0; i < NEQ; i++) Fm[i] = __shfl_up(Fc[i], 1, 64);
+   if (lane >= 2 && live && j >= 1) {
+      const double m = ld<double>(f.m_d[0], p);
+#pragma unroll
+      for (int i = 0; i < NEQ; i++) at<double>(f.Conserved_t, p, i) += (Fc[i] - Fm[i])*m;
+   }
 }
 
 template<class MX, int DIR>
@@ -152,19 +234,22 @@ static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipSt
 {
    const int64_t ncells = exec_cells(g);
    if (ncells == 0) return HTR_OK;
-   const int64_t nfaces = Faces<DIR>::count(g);
-   if (nfaces*MX::NEQ >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many faces");
-   Workspace *wp = workspace(s);
-   if (!wp) return HTR_E_DEVICE;
-   Workspace &w = *wp;
-   int rc = workspace_faces(w, nfaces*MX::NEQ, 0, 0);
-   if (rc) return rc;
-   HTR_LAUNCH(HTR_DIR_NAME("diffusion_face", DIR), (diffusion_face_kernel<MX, DIR>), dim3(grid_for(nfaces, 256)),
-              dim3(256), 0, s, f, mix, g, w.faces, nfaces);
-   if ((rc = check_launch("diffusion_face_kernel"))) return rc;
-   HTR_LAUNCH(HTR_DIR_NAME("diffusion_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>),
-              dim3(grid_for(ncells, 256)), dim3(256), 0, s, f, g, w.faces, nfaces, f.m_d[DIR], ncells);
-   return check_launch("face_divergence_kernel");
+   if constexpr (DIR == 0) {
+      const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XD_OUT - 1)/XD_OUT;
+      const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
+      if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many rows");
+      HTR_LAUNCH("diffusion_x", (diffusion_x_kernel<MX>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0, s, f, mix, g,
+                 (uint32_t)nchunk, (uint32_t)nwaves);
+      return check_launch("diffusion_x_kernel");
+   } else {
+      const int L = walk_segment(DIR);
+      const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
+      const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
+      if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many line segments");
+      HTR_LAUNCH(HTR_DIR_NAME("diffusion", DIR), (diffusion_walk_kernel<MX, DIR>), dim3(grid_for(nunits, 256)),
+                 dim3(256), 0, s, f, mix, g, L, (uint32_t)nseg, (uint32_t)nunits);
+      return check_launch("diffusion_walk_kernel");
+   }
 }
 
 template<class MX>

@@ -6,22 +6,36 @@
 // the three updateRHSSpan loops), src/prometeo_rhs.cu:37-459 (GPU variants).
 //
 // The face flux is a function of its left cell only: the reference's
-// Kennedy-Gruber ring buffer holds pair sums S(a,b) of two cells, so
-// recomputing them per face gives the same values, and its "first minus
-// inter-cell location" is the face of the cell before the first one.  Two
-// traversals, both with consecutive lanes on consecutive x cells:
-//   y and z sweeps: a thread walks a segment of SEG cells along the sweep,
-//     carrying FluxM and the KG pair sums as the reference pencil loop does;
-//   x sweep: a wave walks a list of x rows face by face, lane l holding face
-//     base+l; the minus face of a cell is the flux of lane l-1 (wave shuffle),
-//     lane 0 takes it from lane 63 of the previous round.
-// The TENO path evaluates the characteristic projection one component at a
+// Kennedy-Gruber ring buffer holds pair sums S(a,b) of two cells, and its
+// "first minus inter-cell location" is the face of the cell before the first
+// one (or of the first cell itself at a staggered left boundary).  Passes per
+// direction, all with consecutive lanes on x-adjacent cells:
+//   hybrid: classify faces (sensor) into a compacted TENO list; TENO faces in
+//     three passes (Roe record, characteristic components, back-projection
+//     and limiter) into a face buffer; then the KG walk: one thread per line
+//     segment carries the pair sums and FluxM along the sweep and
+//     differences the fluxes into Conserved_t, reading TENO faces back;
+//   pure TENO family: the three TENO passes over every face, then a
+//     divergence pass over the face buffer;
+//   pure KG: the KG walk alone.
+// The TENO passes evaluate the characteristic projection one component at a
 // time (left-eigenvector rows of rhs.inl:436-505 with the per-point sums
 // hoisted), so no 6 x nEq split-flux arrays are held in registers.
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace htr {
+
+// hybrid sensor of the face of cell p: KG when the shock sensor is clear at
+// pM1, p and pP1 (rhs.inl:818-820)
+template<int DIR>
+__device__ __forceinline__ bool hybrid_kg(const htr_fields &f, const Geo &g, const P3 &p, int t)
+{
+   const P3 pM1 = wminus(g, DIR, p, c_st.Cp[t][1]);
+   const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
+   return ld<uint8_t>(f.shockSensor[DIR], pM1) && ld<uint8_t>(f.shockSensor[DIR], p) &&
+          ld<uint8_t>(f.shockSensor[DIR], pP1);
+}
 
 template<class MX, int DIR>
 struct Euler {
@@ -35,63 +49,6 @@ struct Euler {
 
    __device__ double cons(const P3 &p, int i) const { return ld<double>(f.Conserved, p, i); }
    __device__ double vel(const P3 &p, int i) const { return ld<double>(f.velocity, p, i); }
-
-   // ---- Kennedy-Gruber pair sums S(a,b), b the right cell (rhs.inl:42-75) ----
-   __device__ void pair_sum(double *S, const P3 &a, const P3 &b) const
-   {
-      const double rho_a = ld<double>(f.rho, a), rho_b = ld<double>(f.rho, b);
-      const double pa = ld<double>(f.pressure, a), pb = ld<double>(f.pressure, b);
-      const double H = ld<double>(hv, a);
-      const double rhom = rho_a + rho_b;
-      const double vm = -(vel(a, iN) + vel(b, iN));
-#pragma unroll
-      for (int i = 0; i < NS; i++) S[i] = rhom*vm*(ld<double>(f.MassFracs, a, i) + ld<double>(f.MassFracs, b, i));
-#pragma unroll
-      for (int i = 0; i < 3; i++) S[NS+i] = rhom*vm*(vel(a, i) + vel(b, i));
-      S[NS+3] = rhom*vm*(H + ld<double>(hv, b));
-      S[NEQ] = (pa + pb);
-   }
-
-   // staggered-node KG flux (rhs.inl:92-110)
-   __device__ void kg_staggered(double *Flux, const P3 &p, int t) const
-   {
-      const P3 q = (t == L_S_node) ? p : wplus(g, DIR, p, c_st.Cp[t][2]);
-      const double u = vel(q, iN), pr = ld<double>(f.pressure, q);
-#pragma unroll
-      for (int i = 0; i < NEQ; i++) Flux[i] = -(cons(q, i)*u);
-      Flux[NS+iN] -= pr;
-      Flux[NS+3] -= pr*u;
-   }
-
-   // KGFluxReconstruction (rhs.inl:79-135).  The reference reads ring-buffer
-   // slot (5-m)m/2 + l, which holds S(p-m, p-m+l+1); here each pair sum is
-   // formed when it is added, in the reference's order (l outer, m inner).
-   __device__ void kg_flux(double *Flux, const P3 &p, int t) const
-   {
-      if (t == L_S_node || t == Rm1_S_node) { kg_staggered(Flux, p, t); return; }
-      double fl[NEQ+1];
-#pragma unroll
-      for (int i = 0; i < NEQ+1; i++) fl[i] = 0.0;
-      const int order = c_st.KOrder[t];
-      for (int l = 0; l < order; l++) {
-         double acc[NEQ+1];
-#pragma unroll
-         for (int i = 0; i < NEQ+1; i++) acc[i] = 0.0;
-         for (int mm = 0; mm <= l; mm++) {
-            double S[NEQ+1];
-            const P3 a = wminus(g, DIR, p, -mm);
-            pair_sum(S, a, wplus(g, DIR, a, l+1));
-#pragma unroll
-            for (int i = 0; i < NEQ+1; i++) acc[i] += S[i];
-         }
-         const double c = c_st.KCoeff[t][l];
-#pragma unroll
-         for (int i = 0; i < NEQ+1; i++) fl[i] += c*acc[i];
-      }
-#pragma unroll
-      for (int i = 0; i < NEQ; i++) Flux[i] = 0.25*fl[i];
-      Flux[NS+iN] -= fl[NEQ];
-   }
 
    // ---- Roe state (rhs.inl:200-279) ----------------------------------------
    struct Roe { double a, e, H, a2, rho, v[3], Y[NS], dpde, dpdrhoi[NS]; };
@@ -363,34 +320,6 @@ struct Euler {
       }
    }
 
-   // pass B: component i of the face, F_i = -(F+ + F-)/2 in characteristic
-   // space, and its stencil-selection bits (plus | minus << 4)
-   template<int OP>
-   __device__ double face_component(const double *__restrict__ rec, int64_t cap, int64_t w, const P3 &p,
-                                    int t, int i, uint32_t *bits) const
-   {
-      P3 P[6];
-      points(P, p, t);
-      Roe R;
-      get_rec(rec, cap, w, R);
-      const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
-      Proj J;
-      proj(J, R);
-      const double Ysel = (i >= 1 && i <= NS) ? rec[(i-1)*cap + w] : 0.0;
-      double vp[6], vm[6];
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-         double sums[4];
-         get_sums(rec, cap, w, k, sums);
-         split(vp[k], vm[k], i, P[k], lam, R, J, Ysel, sums);
-      }
-      uint32_t bp, bm;
-      const double FPlus  = teno_reconstruct<OP, 0>(vp, t, &bp);
-      const double FMinus = teno_reconstruct<OP, 1>(vm, t, &bm);
-      *bits = bp | (bm << 4);
-      return -0.5*(FPlus + FMinus);
-   }
-
    // pass C: back-projection (rhs.inl:708) and the positivity limiter
    // (rhs.inl:710-758); returns the decision word
    __device__ uint32_t face_finish(double *Flux, const double *__restrict__ rec, int64_t cap, int64_t w,
@@ -468,13 +397,7 @@ struct Euler {
       return (h << 2) | lim_bit;
    }
 
-   __device__ bool sensor_kg(const P3 &p, int t) const
-   {
-      const P3 pM1 = wminus(g, DIR, p, c_st.Cp[t][1]);
-      const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
-      return ld<uint8_t>(f.shockSensor[DIR], pM1) && ld<uint8_t>(f.shockSensor[DIR], p) &&
-             ld<uint8_t>(f.shockSensor[DIR], pP1);
-   }
+   __device__ bool sensor_kg(const P3 &p, int t) const { return hybrid_kg<DIR>(f, g, p, t); }
 
 };
 
@@ -541,39 +464,364 @@ __device__ inline void store_face(double *__restrict__ F, int64_t nfaces, uint32
    for (int i = 0; i < NEQ; i++) F[i*nfaces + fid] = v[i];
 }
 
-// Pass 1: hybrid sensor and Kennedy-Gruber faces; faces that need TENO are
-// appended to a compacted list (one atomic per wave).  With ALL_KG every
-// face is a KG face (UpdateUsingSkewSymmetricEulerFluxTask).
-template<class MX, int DIR, bool ALL_KG>
+// Pass 1 of the hybrid scheme: the hybrid sensor of every face; faces that
+// need TENO are appended to a compacted list.  A block of 256 threads takes
+// CLS_PER_THREAD*256 consecutive faces and reserves its list range with one
+// atomic (one per wave would serialise some 10^5 atomics on one address).
+constexpr int CLS_PER_THREAD = 16;
+
+template<class MX, int DIR>
 __global__ void __launch_bounds__(256)
-euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, double *__restrict__ F,
-                int64_t nfaces, int32_t *__restrict__ list, int *__restrict__ counter, htr_view dec)
+euler_classify_kernel(const htr_fields f, const Geo g, int64_t nfaces, int32_t *__restrict__ list,
+                      int *__restrict__ counter)
 {
-   const int64_t fid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
-   const bool active = fid < nfaces;
-   bool teno = false;
-   if (active) {
-      const P3 q = Faces<DIR>::cell(g, f, (uint32_t)fid);
-      const int t = ld<int>(f.nType[DIR], q);
-      Euler<MX, DIR> E{f, *mix, g, hv};
-      if (ALL_KG || E.sensor_kg(q, t)) {
-         double Flux[MX::NEQ];
-         E.kg_flux(Flux, q, t);
-         store_face<MX::NEQ>(F, nfaces, (uint32_t)fid, Flux);
-         if (dec.ptr) at<uint32_t>(dec, q) = 1;
+   __shared__ int wave_count[4];
+   __shared__ int block_base;
+   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+   const int64_t tile = (int64_t)blockIdx.x*(256*CLS_PER_THREAD);
+   uint32_t flags = 0;
+   int before = 0;                    // TENO faces of this wave in earlier rounds
+   uint64_t masks[CLS_PER_THREAD];
+#pragma unroll
+   for (int k = 0; k < CLS_PER_THREAD; k++) {
+      const int64_t fid = tile + k*256 + threadIdx.x;
+      bool teno = false;
+      if (fid < nfaces) {
+         const P3 q = Faces<DIR>::cell(g, f, (uint32_t)fid);
+         teno = !hybrid_kg<DIR>(f, g, q, ld<int>(f.nType[DIR], q));
+      }
+      masks[k] = __ballot(teno);
+      if (teno) flags |= 1u << k;
+      before += __popcll(masks[k]);
+   }
+   if (lane == 0) wave_count[wv] = before;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      const int total = wave_count[0] + wave_count[1] + wave_count[2] + wave_count[3];
+      block_base = total ? atomicAdd(counter, total) : 0;
+   }
+   __syncthreads();
+   int base = block_base;
+   for (int w = 0; w < wv; w++) base += wave_count[w];
+   const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+   for (int k = 0; k < CLS_PER_THREAD; k++) {
+      if (flags & (1u << k)) list[base + __popcll(masks[k] & lt)] = (int32_t)(tile + k*256 + threadIdx.x);
+      base += __popcll(masks[k]);
+   }
+}
+
+// ---- Kennedy-Gruber faces walked along the line, divergence fused ----------
+// KGFluxReconstruction (rhs.inl:79-135) needs, at the face of cell p, the
+// pair sums S(p-m, p-m+l+1) for l < KOrder, m <= l: spans 1..3 with left
+// cells p-2..p.  Walking a line in face order, the pairs that start at p
+// (S(p,p+1), S(p,p+2), S(p,p+3)) are new at face p and the others were
+// formed at faces p-1 and p-2, which is the reference's ring buffer of pair
+// sums kept in registers.  The sums enter in the reference's order (l
+// outer, m inner).  The flux of face j is then differenced with that of
+// face j-1 into Conserved_t of the cell between them (rhs.inl:876-887), so
+// no KG face flux goes through memory; TENO faces of the hybrid scheme are
+// read from the face buffer the TENO passes wrote.
+//
+// The conserved components are split in two groups, each walked by its own
+// wave: G = 0 the species (pair values rho Y_i), G = 1 momentum, energy and
+// the pressure sum.  A thread takes one line and one segment of L faces
+// [j0, j0 + L); a segment other than the first also forms face j0 - 1.
+template<class MX, int G>
+struct KGGroup {
+   static constexpr int NS = MX::NS;
+   static constexpr int LO = (G == 0) ? 0 : NS;       // first conserved component of the group
+   static constexpr int NC = (G == 0) ? NS : 4;       // conserved components of the group
+   static constexpr int NP = (G == 0) ? NS : 5;       // pair values (G = 1: + pressure sum)
+};
+
+template<class MX, int DIR, int G>
+struct KGCell {
+   using K = KGGroup<MX, G>;
+   double rho, un, v[K::NP];
+   __device__ void load(const htr_fields &f, const htr_view &hv, const P3 &p)
+   {
+      rho = ld<double>(f.rho, p);
+      un = ld<double>(f.velocity, p, DIR);
+      if constexpr (G == 0) {
+#pragma unroll
+         for (int i = 0; i < K::NS; i++) v[i] = ld<double>(f.MassFracs, p, i);
       } else {
-         teno = true;
+#pragma unroll
+         for (int i = 0; i < 3; i++) v[i] = ld<double>(f.velocity, p, i);
+         v[3] = ld<double>(hv, p);
+         v[4] = ld<double>(f.pressure, p);
       }
    }
-   if (!ALL_KG) {
-      const uint64_t mask = __ballot(teno);
-      if (mask) {
-         const int lane = threadIdx.x & 63;
-         const int leader = __ffsll((unsigned long long)mask) - 1;
-         int base = 0;
-         if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-         base = __shfl(base, leader, 64);
-         if (teno) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)fid;
+};
+
+// pair sum S(a, b) of the group (rhs.inl:42-75)
+template<class MX, int DIR, int G>
+__device__ __forceinline__ void kg_pair(double *S, const KGCell<MX, DIR, G> &a, const KGCell<MX, DIR, G> &b)
+{
+   using K = KGGroup<MX, G>;
+   const double rhom = a.rho + b.rho;
+   const double vm = -(a.un + b.un);
+#pragma unroll
+   for (int i = 0; i < K::NC; i++) S[i] = rhom*vm*(a.v[i] + b.v[i]);
+   if constexpr (G == 1) S[4] = a.v[4] + b.v[4];
+}
+
+template<class MX, int DIR, int G>
+struct KGRing {
+   using K = KGGroup<MX, G>;
+   using Cell = KGCell<MX, DIR, G>;
+   double R2[K::NP], R3a[K::NP], R3b[K::NP];   // S(p-1,p+1), S(p-1,p+2), S(p-2,p+1) at face p
+
+   __device__ void warm(const htr_fields &f, const htr_view &hv, const Geo &g, const P3 &p)
+   {
+      Cell a, b;
+      a.load(f, hv, wminus(g, DIR, p, -1));
+      b.load(f, hv, wplus(g, DIR, p, 1));
+      kg_pair<MX, DIR, G>(R2, a, b);
+      b.load(f, hv, wplus(g, DIR, p, 2));
+      kg_pair<MX, DIR, G>(R3a, a, b);
+      a.load(f, hv, wminus(g, DIR, p, -2));
+      b.load(f, hv, wplus(g, DIR, p, 1));
+      kg_pair<MX, DIR, G>(R3b, a, b);
+   }
+
+   // face of cell p (node type t): KG flux of the group into Flux when kg,
+   // and the ring advanced to face p+1 in any case
+   __device__ void step(const htr_fields &f, const htr_view &hv, const Geo &g, const P3 &p, int t, bool kg,
+                        double *Flux)
+   {
+      Cell a, b;
+      a.load(f, hv, p);
+      double fl[K::NP], N[K::NP];
+      const int order = c_st.KOrder[t];
+      b.load(f, hv, wplus(g, DIR, p, 1));
+      kg_pair<MX, DIR, G>(N, a, b);                                   // l = 0: S(p,p+1)
+      {
+         const double c = c_st.KCoeff[t][0];
+#pragma unroll
+         for (int i = 0; i < K::NP; i++) fl[i] = (order > 0) ? c*N[i] : 0.0;
+      }
+      b.load(f, hv, wplus(g, DIR, p, 2));
+      kg_pair<MX, DIR, G>(N, a, b);                                   // l = 1: S(p,p+2) + S(p-1,p+1)
+      if (order > 1) {
+         const double c = c_st.KCoeff[t][1];
+#pragma unroll
+         for (int i = 0; i < K::NP; i++) fl[i] += c*(N[i] + R2[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < K::NP; i++) R2[i] = N[i];
+      b.load(f, hv, wplus(g, DIR, p, 3));
+      kg_pair<MX, DIR, G>(N, a, b);                                   // l = 2: S(p,p+3) + S(p-1,p+2) + S(p-2,p+1)
+      if (order > 2) {
+         const double c = c_st.KCoeff[t][2];
+#pragma unroll
+         for (int i = 0; i < K::NP; i++) fl[i] += c*((N[i] + R3a[i]) + R3b[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < K::NP; i++) { R3b[i] = R3a[i]; R3a[i] = N[i]; }
+      if (!kg) return;
+      if (t == L_S_node || t == Rm1_S_node) {
+         // staggered-node flux (rhs.inl:92-110)
+         const P3 q = (t == L_S_node) ? p : wplus(g, DIR, p, c_st.Cp[t][2]);
+         const double u = ld<double>(f.velocity, q, DIR);
+#pragma unroll
+         for (int i = 0; i < K::NC; i++) Flux[i] = -(ld<double>(f.Conserved, q, K::LO + i)*u);
+         if constexpr (G == 1) {
+            const double pr = ld<double>(f.pressure, q);
+            Flux[DIR] -= pr;
+            Flux[3] -= pr*u;
+         }
+         return;
+      }
+#pragma unroll
+      for (int i = 0; i < K::NC; i++) Flux[i] = 0.25*fl[i];
+      if constexpr (G == 1) Flux[DIR] -= fl[4];
+   }
+};
+
+template<class MX, int DIR, bool ALL_KG, int G>
+__device__ void kg_walk(const htr_fields &f, const Geo &g, const htr_view &hv, const double *__restrict__ F,
+                        int64_t nfaces, uint32_t line, int j0, int j1, const htr_view &dec)
+{
+   using K = KGGroup<MX, G>;
+   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B;
+   int c[3];
+   c[A] = g.elo[A] + (int)(line % (uint32_t)g.esz[A]);
+   c[B] = g.elo[B] + (int)(line / (uint32_t)g.esz[A]);
+   c[DIR] = g.elo[DIR];
+   const P3 p0 = mkp(g, c[0], c[1], c[2]);
+   KGRing<MX, DIR, G> ring;
+   double Fm[K::NC], Fc[K::NC];
+   auto face = [&](const P3 &p, int t, int j, double *Fl) {
+      const bool kg = ALL_KG || hybrid_kg<DIR>(f, g, p, t);
+      ring.step(f, hv, g, p, t, kg, Fl);
+      if (!kg) {
+         const uint32_t fid = Faces<DIR>::id(g, line, j);
+#pragma unroll
+         for (int i = 0; i < K::NC; i++) Fl[i] = F[(int64_t)(K::LO + i)*nfaces + fid];
+      } else if (G == 0 && dec.ptr) {
+         at<uint32_t>(dec, p) = 1;
+      }
+   };
+   int j;
+   P3 p;
+   if (j0 == 0) {
+      // face 0 at p0 + offM1(nType[p0]) (rhs.inl:800-817)
+      const P3 q = wminus(g, DIR, p0, c_st.Cp[ld<int>(f.nType[DIR], p0)][1]);
+      ring.warm(f, hv, g, q);
+      face(q, ld<int>(f.nType[DIR], q), 0, Fm);
+      j = 1;
+      p = p0;
+   } else {
+      j = j0 - 1;
+      p = wplus(g, DIR, p0, j - 1);
+   }
+   ring.warm(f, hv, g, p);
+   if (j0 > 0) {
+      face(p, ld<int>(f.nType[DIR], p), j, Fm);
+      j++;
+      p = wplus(g, DIR, p, 1);
+   }
+   for (; j < j1; j++) {
+      face(p, ld<int>(f.nType[DIR], p), j, Fc);
+      const double m = ld<double>(f.m_e[DIR], p);
+#pragma unroll
+      for (int i = 0; i < K::NC; i++) {
+         at<double>(f.Conserved_t, p, K::LO + i) += (Fc[i] - Fm[i])*m;
+         Fm[i] = Fc[i];
+      }
+      p = wplus(g, DIR, p, 1);
+   }
+}
+
+// Work units: (line, segment) pairs, nseg segments of L faces per line; for
+// the x sweep consecutive units are consecutive segments of a line, for y
+// and z consecutive lines (x-adjacent cells).  One launch per component
+// group G.
+template<class MX, int DIR, bool ALL_KG, int G>
+__global__ void __launch_bounds__(256)
+euler_kg_walk_kernel(const htr_fields f, const Geo g, const htr_view hv, const double *__restrict__ F,
+                     int64_t nfaces, int L, uint32_t nseg, uint32_t nunits, htr_view dec)
+{
+   const uint32_t u = blockIdx.x*blockDim.x + threadIdx.x;
+   if (u >= nunits) return;
+   uint32_t line, seg;
+   if (DIR == 0) { line = u/nseg; seg = u - line*nseg; }
+   else {
+      const uint32_t nl = (uint32_t)g.esz[Faces<DIR>::A]*(uint32_t)g.esz[Faces<DIR>::B];
+      seg = u/nl; line = u - seg*nl;
+   }
+   const int j0 = (int)seg*L;
+   const int j1 = min(j0 + L, g.esz[DIR] + 1);
+   kg_walk<MX, DIR, ALL_KG, G>(f, g, hv, F, nfaces, line, j0, j1, dec);
+}
+
+// x sweep: lanes on consecutive faces of one row.  A wave takes XK_OUT = 61
+// faces (lanes 3..63) and forms the new pairs S(p,p+d) of its 64 cells; a
+// lane's older pairs are those of lanes l-1 and l-2 and its FluxM is the flux
+// of lane l-1 (wave shuffles), so lanes 0..2 serve as the halo of lane 3.
+constexpr int XK_OUT = 61;
+
+template<class MX, bool ALL_KG, int G>
+__global__ void __launch_bounds__(256)
+euler_kg_x_kernel(const htr_fields f, const Geo g, const htr_view hv, const double *__restrict__ F,
+                  int64_t nfaces, uint32_t nchunk, uint32_t nwaves, htr_view dec)
+{
+   using K = KGGroup<MX, G>;
+   using Cell = KGCell<MX, 0, G>;
+   const uint32_t wave = (blockIdx.x*blockDim.x + threadIdx.x) >> 6;
+   if (wave >= nwaves) return;                                      // whole waves
+   const int lane = threadIdx.x & 63;
+   const uint32_t row = wave/nchunk, k = wave - row*nchunk;
+   const int esz = g.esz[0];
+   const int j = (int)k*XK_OUT + lane - 3;                           // face of this lane
+   const P3 prow = mkp(g, g.elo[0], g.elo[1] + (int)(row % (uint32_t)g.esz[1]),
+                       g.elo[2] + (int)(row/(uint32_t)g.esz[1]));
+   const P3 p = wplus(g, 0, prow, j - 1);                            // left cell of face j
+   const bool live = j <= esz;
+   double N1[K::NP], N2[K::NP], N3[K::NP];
+   int t = 0;
+   if (live) {
+      Cell a, b;
+      a.load(f, hv, p);
+      t = ld<int>(f.nType[0], p);
+      b.load(f, hv, wplus(g, 0, p, 1));
+      kg_pair<MX, 0, G>(N1, a, b);
+      b.load(f, hv, wplus(g, 0, p, 2));
+      kg_pair<MX, 0, G>(N2, a, b);
+      b.load(f, hv, wplus(g, 0, p, 3));
+      kg_pair<MX, 0, G>(N3, a, b);
+   } else {
+#pragma unroll
+      for (int i = 0; i < K::NP; i++) { N1[i] = 0.0; N2[i] = 0.0; N3[i] = 0.0; }
+   }
+   double R2[K::NP], R3a[K::NP], R3b[K::NP];
+#pragma unroll
+   for (int i = 0; i < K::NP; i++) {
+      R2[i] = __shfl_up(N2[i], 1, 64);
+      R3a[i] = __shfl_up(N3[i], 1, 64);
+      R3b[i] = __shfl_up(N3[i], 2, 64);
+   }
+   double Fc[K::NC];
+#pragma unroll
+   for (int i = 0; i < K::NC; i++) Fc[i] = 0.0;
+   bool kg = false;
+   P3 q = p;                                                          // the face's cell
+   if (lane >= 2 && j >= 0 && live) {
+      int tq = t;
+      if (j == 0) {
+         // face 0 at p0 + offM1(nType[p0]) (rhs.inl:800-817): the natural
+         // left cell except at a staggered left boundary
+         const P3 p0 = wplus(g, 0, p, 1);
+         q = wminus(g, 0, p0, c_st.Cp[ld<int>(f.nType[0], p0)][1]);
+         tq = ld<int>(f.nType[0], q);
+      }
+      kg = ALL_KG || hybrid_kg<0>(f, g, q, tq);
+      if (!kg) {
+         const uint32_t fid = row*(uint32_t)(esz + 1) + (uint32_t)j;
+#pragma unroll
+         for (int i = 0; i < K::NC; i++) Fc[i] = F[(int64_t)(K::LO + i)*nfaces + fid];
+      } else if (q.o != p.o) {
+         KGRing<MX, 0, G> ring;
+         ring.warm(f, hv, g, q);
+         ring.step(f, hv, g, q, tq, true, Fc);
+      } else if (t == L_S_node || t == Rm1_S_node) {
+         const P3 qs = (t == L_S_node) ? p : wplus(g, 0, p, c_st.Cp[t][2]);
+         const double u = ld<double>(f.velocity, qs, 0);
+#pragma unroll
+         for (int i = 0; i < K::NC; i++) Fc[i] = -(ld<double>(f.Conserved, qs, K::LO + i)*u);
+         if constexpr (G == 1) {
+            const double pr = ld<double>(f.pressure, qs);
+            Fc[0] -= pr;
+            Fc[3] -= pr*u;
+         }
+      } else {
+         // KGFluxReconstruction (rhs.inl:79-135), l outer, m inner
+         const int order = c_st.KOrder[t];
+         double fl[K::NP];
+         const double c0 = c_st.KCoeff[t][0], c1 = c_st.KCoeff[t][1], c2 = c_st.KCoeff[t][2];
+#pragma unroll
+         for (int i = 0; i < K::NP; i++) {
+            double v = (order > 0) ? c0*N1[i] : 0.0;
+            if (order > 1) v += c1*(N2[i] + R2[i]);
+            if (order > 2) v += c2*((N3[i] + R3a[i]) + R3b[i]);
+            fl[i] = v;
+         }
+#pragma unroll
+         for (int i = 0; i < K::NC; i++) Fc[i] = 0.25*fl[i];
+         if constexpr (G == 1) Fc[0] -= fl[4];
+      }
+   }
+   double Fm[K::NC];
+#pragma unroll
+   for (int i = 0; i < K::NC; i++) Fm[i] = __shfl_up(Fc[i], 1, 64);
+   if (lane >= 3 && live && j >= 0) {
+      if (G == 0 && kg && dec.ptr) at<uint32_t>(dec, q) = 1;
+      if (j >= 1) {
+         const double m = ld<double>(f.m_e[0], p);
+#pragma unroll
+         for (int i = 0; i < K::NC; i++) at<double>(f.Conserved_t, p, K::LO + i) += (Fc[i] - Fm[i])*m;
       }
    }
 }
@@ -581,7 +829,8 @@ euler_kg_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, c
 // Pass 2: TENO-family faces, either all faces or the compacted list, in
 // three kernels that each keep a small register footprint:
 //   2a  lane per face: Roe state + eigenvalue maxima -> record
-//   2b  lane per (component, face): split fluxes, TENO +/- -> component, bits
+//   2b  a wave per (component, 64 faces), face data staged in LDS: split
+//       fluxes, TENO +/- -> component, bits
 //   2c  lane per face: back-projection, limiter -> face flux, decision word
 // Buffers (capacity cap = number of faces): record NREC*cap doubles, then
 // components NEQ*cap doubles, then bits NEQ*cap bytes.
@@ -621,35 +870,6 @@ euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, 
    }
 }
 
-template<class MX, int DIR, int SCHEME, bool LISTED>
-__global__ void __launch_bounds__(256)
-euler_char_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b, int64_t nfaces,
-                  const int32_t *__restrict__ list, const int *__restrict__ counter)
-{
-   constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
-   constexpr uint32_t CH = 64*MX::NEQ;          // items of one chunk: 64 faces x nEq components
-   const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
-   const uint32_t nitems = ((n + 63u)/64u)*CH;
-   const uint32_t stride = gridDim.x*blockDim.x;
-   Euler<MX, DIR> E{f, *mix, g, hv};
-   // a wave takes one component of 64 consecutive faces, and the nEq waves of
-   // a chunk are adjacent items, so a face's stencil data and record are read
-   // from HBM once and served from cache to its other components
-   for (uint32_t it = blockIdx.x*blockDim.x + threadIdx.x; it < nitems; it += stride) {
-      const uint32_t chunk = it/CH;
-      const uint32_t r = it - chunk*CH;
-      const uint32_t i = r >> 6;
-      const uint32_t w = chunk*64u + (r & 63u);
-      if (w >= n) continue;
-      const uint32_t fid = LISTED ? (uint32_t)list[w] : w;
-      const P3 q = Faces<DIR>::cell(g, f, fid);
-      uint32_t bits;
-      const double fc = E.template face_component<OP>(b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q), (int)i, &bits);
-      b.Fc[(int64_t)i*b.cap + w] = fc;
-      b.bits[(int64_t)i*b.cap + w] = (uint8_t)bits;
-   }
-}
-
 // Pass 2b with the face data staged in LDS: a block of nEq waves takes a
 // chunk of 64 TENO faces; all its threads load, with coalesced reads, the
 // values the components need (per face: the Roe state, the eigenvalue
@@ -668,7 +888,7 @@ struct CharStage {
 };
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
-__global__ void __launch_bounds__(64*MX::NEQ)
+__global__ void __launch_bounds__(64*MX::NEQ) __attribute__((amdgpu_waves_per_eu(SCHEME == HTR_EULER_TENOA ? 3 : 5)))
 euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b,
                       int64_t nfaces, const int32_t *__restrict__ list, const int *__restrict__ counter)
 {
@@ -842,7 +1062,7 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    Workspace *wp = workspace(s);
    if (!wp) return HTR_E_DEVICE;
    Workspace &w = *wp;
-   int rc = workspace_faces(w, nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0,
+   int rc = workspace_faces(w, SCHEME == HTR_EULER_KG ? 0 : nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0,
                             SCHEME == HTR_EULER_KG ? 0 : teno_doubles<MX>(nfaces));
    if (rc) return rc;
    // total enthalpy of every cell of the bounds
@@ -855,22 +1075,46 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    hv.sx = 8;                                  // one double per base element (k = 1)
    HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
    if ((rc = check_launch("enthalpy_kernel"))) return rc;
-   if constexpr (SCHEME == HTR_EULER_KG) {
-      HTR_LAUNCH(HTR_DIR_NAME("euler_kg", DIR), (euler_kg_kernel<MX, DIR, true>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
-                         f, mix, g, hv, w.faces, nfaces, w.list, w.counter + 1, dec);
-      if ((rc = check_launch("euler_kg_kernel"))) return rc;
-   } else if constexpr (SCHEME == HTR_EULER_HYBRID) {
+   if constexpr (SCHEME == HTR_EULER_HYBRID) {
       if ((rc = zero_counters(w.counter, s))) return rc;
-      HTR_LAUNCH(HTR_DIR_NAME("euler_kg", DIR), (euler_kg_kernel<MX, DIR, false>), dim3(grid_for(nfaces, 256)), dim3(256), 0, s,
-                         f, mix, g, hv, w.faces, nfaces, w.list, w.counter + 1, dec);
-      if ((rc = check_launch("euler_kg_kernel"))) return rc;
+      HTR_LAUNCH(HTR_DIR_NAME("euler_classify", DIR), (euler_classify_kernel<MX, DIR>),
+                 dim3(grid_for(nfaces, 256*CLS_PER_THREAD)),
+                 dim3(256), 0, s, f, g, nfaces, w.list, w.counter + 1);
+      if ((rc = check_launch("euler_classify_kernel"))) return rc;
       if ((rc = launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
+   }
+   if constexpr (SCHEME == HTR_EULER_HYBRID || SCHEME == HTR_EULER_KG) {
+      constexpr bool ALL = SCHEME == HTR_EULER_KG;
+      if constexpr (DIR == 0) {
+         const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XK_OUT - 1)/XK_OUT;
+         const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
+         if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many rows");
+         HTR_LAUNCH("euler_kg_species_x", (euler_kg_x_kernel<MX, ALL, 0>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
+                    s, f, g, hv, w.faces, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
+         if ((rc = check_launch("euler_kg_x_kernel"))) return rc;
+         HTR_LAUNCH("euler_kg_flow_x", (euler_kg_x_kernel<MX, ALL, 1>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
+                    s, f, g, hv, w.faces, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
+         return check_launch("euler_kg_x_kernel");
+      } else {
+         const int L = walk_segment(DIR);
+         const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
+         const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
+         if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many line segments");
+         HTR_LAUNCH(HTR_DIR_NAME("euler_kg_species", DIR), (euler_kg_walk_kernel<MX, DIR, ALL, 0>),
+                    dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, g, hv, w.faces, nfaces, L, (uint32_t)nseg,
+                    (uint32_t)nunits, dec);
+         if ((rc = check_launch("euler_kg_walk_kernel"))) return rc;
+         HTR_LAUNCH(HTR_DIR_NAME("euler_kg_flow", DIR), (euler_kg_walk_kernel<MX, DIR, ALL, 1>),
+                    dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, g, hv, w.faces, nfaces, L, (uint32_t)nseg,
+                    (uint32_t)nunits, dec);
+         return check_launch("euler_kg_walk_kernel");
+      }
    } else {
       if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
+      HTR_LAUNCH(HTR_DIR_NAME("euler_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)),
+                 dim3(256), 0, s, f, g, w.faces, nfaces, f.m_e[DIR], ncells);
+      return check_launch("face_divergence_kernel");
    }
-   HTR_LAUNCH(HTR_DIR_NAME("euler_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)), dim3(256), 0, s,
-                      f, g, w.faces, nfaces, f.m_e[DIR], ncells);
-   return check_launch("face_divergence_kernel");
 }
 
 template<class MX>

@@ -70,4 +70,21 @@ void prof_end(hipStream_t s);
 // "name_x", "name_y", "name_z"
 #define HTR_DIR_NAME(base, dir) ((dir) == 0 ? base "_x" : (dir) == 1 ? base "_y" : base "_z")
 
+// Faces per segment of the y and z line walks (KG and diffusion); the x
+// sweeps use wave-wide rows instead and ignore entry 0.  HTR_WALK_SEGMENT=
+// "Lx,Ly,Lz" overrides (tuning).
+static inline int walk_segment(int dir)
+{
+   static int seg[3] = { 0, 0, 0 };
+   if (seg[0] == 0) {
+      int v[3] = { 1, 32, 32 };
+      if (const char *e = getenv("HTR_WALK_SEGMENT")) {
+         int a, b, c;
+         if (sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && a > 0 && b > 0 && c > 0) { v[0] = a; v[1] = b; v[2] = c; }
+      }
+      seg[1] = v[1]; seg[2] = v[2]; seg[0] = v[0];
+   }
+   return seg[dir];
+}
+
 } // namespace htr
