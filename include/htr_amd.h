@@ -25,9 +25,11 @@
  *     except htr_max_spectral_radius, which returns the maximum to the host
  *     (the reference returns a future), and htr_chem_update, which reads its
  *     failure count back;
- *   - device workspace: allocated once per device by htr_init; the Euler
- *     face buffers grow (only) at the first call on a larger rect, so a
- *     solver's first step allocates and later steps do not;
+ *   - device workspace: one per (device, stream), created at the first call
+ *     on that stream; the Euler TENO face buffers grow (only) at the first
+ *     call on a larger rect, so a solver's first step allocates and later
+ *     steps do not (a growth waits for the stream's earlier work before it
+ *     frees the old buffer); calls on different streams may overlap;
  *   - every function returns 0 on success or a negative HTR_E_* code;
  *     htr_last_error() describes the last failure of the calling thread;
  *   - mixtures are uploaded once (htr_mix_*), as LoadMixtureTask does.
