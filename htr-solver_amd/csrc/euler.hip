@@ -179,12 +179,11 @@ struct Euler {
    }
 
    // ---- FluxReconstruction<Op> (rhs.inl:625-759) in three passes ----------
-   // face record: Roe state, the Lax-Friedrichs eigenvalue maxima, and per
-   // stencil point k the species sums of the left-eigenvector products of the
-   // conserved vector and of the Euler flux (sum_j w_j q_j, sum_j q_j,
-   // sum_j w_j q_j u_n, sum_j q_j u_n with w_j = (Coeff - dpdrhoi_j)/a^2)
-   static constexpr int NREC = 2*NS + 12 + 24;
-   static constexpr int RSUM = 2*NS + 12;        // first sum slot: RSUM + 4k + {0,1,2,3}
+   // face record: Roe state and the Lax-Friedrichs eigenvalue maxima.  The
+   // per-point species sums of the left-eigenvector products (point_sums)
+   // are formed where the point values are staged (the component pass) or,
+   // on the limiter path, in the back-projection pass.
+   static constexpr int NREC = 2*NS + 12;
 
    // stencil points pM2..pP3 of the face at p (rhs.inl:640-655)
    __device__ __forceinline__ void points(P3 *P, const P3 &p, int t) const
@@ -292,12 +291,6 @@ struct Euler {
       vm = F - lam*Q;
    }
 
-   __device__ static void get_sums(const double *__restrict__ rec, int64_t cap, int64_t w, int k, double *sums)
-   {
-#pragma unroll
-      for (int c = 0; c < 4; c++) sums[c] = rec[(RSUM + 4*k + c)*cap + w];
-   }
-
    // pass A: Roe state and eigenvalue maxima of face p
    __device__ void face_record(double *__restrict__ rec, int64_t cap, int64_t w, const P3 &p, int t) const
    {
@@ -308,16 +301,6 @@ struct Euler {
       Roe R;
       roe(R, p, P[3]);
       put_rec(rec, cap, w, R, Lam1, Lam, LamN);
-      Proj J;
-      proj(J, R);
-#pragma unroll 1
-      for (int k = 0; k < 6; k++) {
-         double sums[4];
-         const P3 Pk = (k == 2) ? p : (k < 2) ? wminus(g, DIR, p, c_st.Cp[t][k]) : wplus(g, DIR, p, c_st.Cp[t][k-1]);
-         point_sums(Pk, R, J, sums);
-#pragma unroll
-         for (int c = 0; c < 4; c++) rec[(RSUM + 4*k + c)*cap + w] = sums[c];
-      }
    }
 
    // pass C: back-projection (rhs.inl:708) and the positivity limiter
@@ -356,14 +339,14 @@ struct Euler {
             Proj J;
             proj(J, R);
             double FLO[NEQ];
-            double sq = 0.0;
+            double sq = 0.0, s2[4], s3[4];
+            point_sums(p, R, J, s2);
+            point_sums(pP1, R, J, s3);
 #pragma unroll
             for (int i = 0; i < NEQ; i++) {
                const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
                const double Ysel = (i >= 1 && i <= NS) ? R.Y[i-1] : 0.0;
-               double ap, am, bp, bm, s2[4], s3[4];
-               get_sums(rec, cap, w, 2, s2);
-               get_sums(rec, cap, w, 3, s3);
+               double ap, am, bp, bm;
                split(ap, am, i, p, lam, R, J, Ysel, s2);
                split(bp, bm, i, pP1, lam, R, J, Ysel, s3);
                add_column(FLO, sq, i, -0.5*(ap + bm), R, ERoe, dedp);
@@ -880,71 +863,121 @@ template<class MX>
 struct CharStage {
    static constexpr int NS = MX::NS, NEQ = MX::NEQ;
    static constexpr int NPV = NS + 6;                // per point: rhoY_0..NS-1, rhou 3, rhoE, u_n, p
-   static constexpr int NRV = NS + 35;               // per face: Y NS, v 3, a, e, rho, dpde, a2, Lam 3, sums 24
+   static constexpr int NRV = 2*NS + 11;             // per face: Y NS, v 3, a, e, rho, dpde, a2, Lam 3, dpdrhoi NS
+   static constexpr int MV = (NPV + NEQ - 1)/NEQ;    // point values per thread (v = wave + m*nEq)
+   static constexpr int MR = (NRV + NEQ - 1)/NEQ;    // record values per thread
    double pt[6*NPV][64];
    double rc[NRV][64];
+   double sm[24][64];                                // per point k: the four species sums 4k..4k+3
    int    t[64];
-   uint32_t fid[64];
 };
 
+// record slot of staged value r: Y 0..NS-1 -> rec 0..NS-1; v, a -> NS..NS+3;
+// e, rho, dpde -> NS+5..NS+7 (H skipped); a2; Lam1, Lam, LamN; dpdrhoi
+template<class MX>
+__device__ __forceinline__ int char_rec_slot(int r)
+{
+   constexpr int NS = MX::NS;
+   if (r < NS + 4) return r;
+   if (r < NS + 7) return r + 1;
+   if (r < NS + 8) return 2*NS + 8;
+   if (r < NS + 11) return 2*NS + 9 + (r - NS - 8);
+   return NS + 8 + (r - NS - 11);
+}
+
 template<class MX, int DIR, int SCHEME, bool LISTED>
-__global__ void __launch_bounds__(64*MX::NEQ) __attribute__((amdgpu_waves_per_eu(SCHEME == HTR_EULER_TENOA ? 3 : 5)))
+__global__ void __launch_bounds__(64*MX::NEQ)
 euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b,
                       int64_t nfaces, const int32_t *__restrict__ list, const int *__restrict__ counter)
 {
    using E_t = Euler<MX, DIR>;
-   constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = CharStage<MX>::NPV, NRV = CharStage<MX>::NRV;
+   using St = CharStage<MX>;
+   constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = St::NPV, NRV = St::NRV, MV = St::MV, MR = St::MR;
    constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
    constexpr int iN = E_t::iN;
-   __shared__ CharStage<MX> S;
+   __shared__ St S;
    const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
    const uint32_t nchunks = (n + 63u)/64u;
    const int lane = threadIdx.x & 63;
    const int wv = threadIdx.x >> 6;                   // wave = component
    E_t E{f, *mix, g, hv};
-   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+   // Stencil-point values of a chunk, fetched into registers one chunk ahead
+   // so that their scattered loads are in flight while the previous chunk
+   // computes: every wave forms its lane's face geometry and loads a share
+   // of the point values (the geometry loads first: the loads' counter
+   // completes in order).  The record values, read with coalesced loads,
+   // are staged when the chunk is deposited.
+   // (TENO-A keeps no values across the compute: its reconstruction needs
+   // the registers, so it stages each chunk straight into LDS)
+   constexpr bool PF = SCHEME != HTR_EULER_TENOA;
+   double pv[PF ? MV : 1][6];
+   int tv = 0;
+   auto fetch = [&](uint32_t c) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
-      // ---- stage: every wave computes its lane's face geometry, then loads
-      // a share of the point values and of the record
       P3 q = mkp(g, 0, 0, 0);
       int t = 0;
       if (valid) {
-         const uint32_t fd = LISTED ? (uint32_t)list[w] : w;
-         q = Faces<DIR>::cell(g, f, fd);
+         q = Faces<DIR>::cell(g, f, LISTED ? (uint32_t)list[w] : w);
          t = ld<int>(f.nType[DIR], q);
-         if (wv == 0) { S.t[lane] = t; S.fid[lane] = fd; }
       }
-      for (int v = wv; v < NPV; v += NEQ) {
+      tv = t;                                     // Std_node (0) on lanes past the end
+#pragma unroll
+      for (int m = 0; m < MV; m++) {
+         const int v = wv + m*NEQ;
 #pragma unroll
          for (int k = 0; k < 6; k++) {
             double x = 0.0;
-            if (valid) {
+            if (valid && v < NPV) {
                const P3 P = (k == 2) ? q : (k < 2) ? wminus(g, DIR, q, c_st.Cp[t][k]) : wplus(g, DIR, q, c_st.Cp[t][k-1]);
                x = (v < NS + 4) ? ld<double>(f.Conserved, P, v) : (v == NS + 4) ? ld<double>(f.velocity, P, iN)
                                                                                   : ld<double>(f.pressure, P);
             }
-            S.pt[k*NPV + v][lane] = x;
+            if constexpr (PF) pv[m][k] = x;
+            else if (v < NPV) S.pt[k*NPV + v][lane] = x;
          }
       }
-      for (int r = wv; r < NRV; r += NEQ) {
-         // record field of slot r: Y 0..NS-1 -> rec 0..NS-1; v, a, H, e, rho, dpde -> NS..NS+7 minus H;
-         int src;
-         if (r < NS) src = r;                                   // Y
-         else if (r < NS + 4) src = r;                          // v[3], a
-         else if (r < NS + 7) src = r + 1;                      // e, rho, dpde (skip H)
-         else if (r < NS + 8) src = 2*NS + 8;                   // a2
-         else if (r < NS + 11) src = 2*NS + 9 + (r - NS - 8);   // Lam1, Lam, LamN
-         else src = E_t::RSUM + (r - NS - 11);                  // sums
-         S.rc[r][lane] = valid ? b.rec[(int64_t)src*b.cap + w] : 0.0;
+      if constexpr (!PF) {
+#pragma unroll
+         for (int m = 0; m < MR; m++) {
+            const int r = wv + m*NEQ;
+            if (r < NRV) S.rc[r][lane] = valid ? b.rec[(int64_t)char_rec_slot<MX>(r)*b.cap + w] : 0.0;
+         }
+      }
+   };
+   if (PF && blockIdx.x < nchunks) fetch(blockIdx.x);
+   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+      const uint32_t w = c*64u + (uint32_t)lane;
+      const bool valid = w < n;
+      // ---- deposit the fetched chunk and its record, then start fetching
+      // the next chunk
+      if constexpr (!PF) fetch(c);
+      else {
+#pragma unroll
+         for (int m = 0; m < MR; m++) {
+            const int r = wv + m*NEQ;
+            if (r < NRV) S.rc[r][lane] = valid ? b.rec[(int64_t)char_rec_slot<MX>(r)*b.cap + w] : 0.0;
+         }
+      }
+      if (wv == 0) S.t[lane] = tv;
+      if constexpr (PF) {
+#pragma unroll
+         for (int m = 0; m < MV; m++) {
+            const int v = wv + m*NEQ;
+            if (v < NPV) {
+#pragma unroll
+               for (int k = 0; k < 6; k++) S.pt[k*NPV + v][lane] = pv[m][k];
+            }
+         }
       }
       __syncthreads();
-      // ---- compute: wave wv = component i of the 64 faces
-      if (valid && wv < NEQ) {
-         const int i = wv;
-         typename E_t::Roe R;
+      if (PF && c + gridDim.x < nchunks) fetch(c + gridDim.x);
+      // ---- the species sums of point k of the 64 faces (wave k, then k +
+      // nEq), the per-point part of projectToCharacteristicSpace
+      // (rhs.inl:443-451)
+      auto roe_of = [&](typename E_t::Roe &R) {
 #pragma unroll
-         for (int s = 0; s < NS; s++) { R.Y[s] = S.rc[s][lane]; R.dpdrhoi[s] = 0.0; }
+         for (int s = 0; s < NS; s++) { R.Y[s] = S.rc[s][lane]; R.dpdrhoi[s] = S.rc[NS + 11 + s][lane]; }
 #pragma unroll
          for (int j = 0; j < 3; j++) R.v[j] = S.rc[NS + j][lane];
          R.a = S.rc[NS + 3][lane];
@@ -953,6 +986,36 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
          R.rho = S.rc[NS + 5][lane];
          R.dpde = S.rc[NS + 6][lane];
          R.a2 = S.rc[NS + 7][lane];
+      };
+      for (int k = __builtin_amdgcn_readfirstlane(wv); valid && k < 6; k += NEQ) {
+         typename E_t::Roe R;
+         roe_of(R);
+         typename E_t::Proj J;
+         E_t::proj(J, R);
+         const double un = S.pt[k*NPV + NS + 4][lane];
+         double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0;
+#pragma unroll
+         for (int j = 0; j < NS; j++) {
+            const double q = S.pt[k*NPV + j][lane];
+            const double wj = (J.Coeff - R.dpdrhoi[j])*J.iaRoe2;
+            s1q += wj*q;
+            sqq += q;
+            const double fq = q*un;
+            s1f += wj*fq;
+            sqf += fq;
+         }
+         S.sm[4*k + 0][lane] = s1q; S.sm[4*k + 1][lane] = sqq;
+         S.sm[4*k + 2][lane] = s1f; S.sm[4*k + 3][lane] = sqf;
+      }
+      __syncthreads();
+      // ---- compute: wave wv = component i of the 64 faces (i in a scalar
+      // register, so every branch on it is uniform)
+      const int tt = S.t[lane];
+      const bool all_std = __ballot(tt != Std_node) == 0ull;
+      if (valid && wv < NEQ) {
+         const int i = __builtin_amdgcn_readfirstlane(wv);
+         typename E_t::Roe R;
+         roe_of(R);
          const double lam = S.rc[NS + 8 + ((i == 0) ? 0 : (i == NEQ-1) ? 2 : 1)][lane];
          typename E_t::Proj J;
          E_t::proj(J, R);
@@ -963,7 +1026,7 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
             const double *pv = &S.pt[k*NPV][0];
             double sums[4];
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++) sums[cc] = S.rc[NS + 11 + 4*k + cc][lane];
+            for (int cc = 0; cc < 4; cc++) sums[cc] = S.sm[4*k + cc][lane];
             const double un = S.pt[k*NPV + NS + 4][lane];
             const double pr = S.pt[k*NPV + NS + 5][lane];
             const double qs = (i >= 1 && i <= NS) ? S.pt[k*NPV + i - 1][lane] : 0.0;
@@ -979,9 +1042,14 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
             vm[k] = F - lam*Q;
          }
          uint32_t bp, bm;
-         const int tt = S.t[lane];
-         const double FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
-         const double FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
+         double FPlus, FMinus;
+         if (OP != 1 && all_std) {
+            FPlus  = teno_reconstruct<OP, 0, true>(vp, tt, &bp);
+            FMinus = teno_reconstruct<OP, 1, true>(vm, tt, &bm);
+         } else {
+            FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
+            FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
+         }
          b.Fc[(int64_t)i*b.cap + w] = -0.5*(FPlus + FMinus);
          b.bits[(int64_t)i*b.cap + w] = (uint8_t)(bp | (bm << 4));
       }

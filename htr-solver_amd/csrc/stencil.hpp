@@ -29,6 +29,20 @@ struct StencilTables {
    float  p10[14];
 };
 
+// Candidate-stencil rows over [c-2 .. c+3] and the optimal weights of the
+// interior node type (src/prometeo_metric_coeffs_macros.h); tables.hip builds
+// every node type's table from these, and the TENO reconstruction uses them as
+// immediates when a whole wave sits on Std_node faces.
+namespace rows {
+constexpr double s1p[6] = { 0.0, -1.0/6.0, 5.0/6.0, 2.0/6.0, 0.0, 0.0 };
+constexpr double s2p[6] = { 0.0, 0.0, 2.0/6.0, 5.0/6.0, -1.0/6.0, 0.0 };
+constexpr double s3p[6] = { 2.0/6.0, -7.0/6.0, 11.0/6.0, 0.0, 0.0, 0.0 };
+constexpr double s4p[6] = { 0.0, 0.0, 3.0/12.0, 13.0/12.0, -5.0/12.0, 1.0/12.0 };
+constexpr double s3m[6] = { 0.0, 0.0, 0.0, 11.0/6.0, -7.0/6.0, 2.0/6.0 };
+constexpr double s4m[6] = { 1.0/12.0, -5.0/12.0, 13.0/12.0, 3.0/12.0, 0.0, 0.0 };
+constexpr double std4[4] = { 9.0/20.0, 6.0/20.0, 1.0/20.0, 4.0/20.0 };
+} // namespace rows
+
 extern __constant__ StencilTables c_st;
 void upload_stencil_tables();            // host: once per device (htr_init)
 const StencilTables &host_stencil_tables();
@@ -79,11 +93,16 @@ __device__ __forceinline__ void smooth_minus(const double *v, double *s)
 
 // OP 0: TENO_Op<-10>, 1: TENOA_Op<-8>, 2: TENOLAD_Op<-16>; SIDE 0 plus, 1 minus.
 // Returns the reconstructed value; *bits receives b1 | b2<<1 | b3<<2 | b4<<3.
-template<int OP, int SIDE>
+// STD: the caller guarantees t == Std_node; the stencil rows and weights are
+// then immediates instead of per-lane table loads.
+template<int OP, int SIDE, bool STD = false>
 __device__ __forceinline__ double teno_reconstruct(const double *v, int t, uint32_t *bits)
 {
-   const double *Coeffs = SIDE ? c_st.CoeffM[t] : c_st.CoeffP[t];
+   const double *Coeffs = STD ? rows::std4 : SIDE ? c_st.CoeffM[t] : c_st.CoeffP[t];
+   const double *const std_rows[4] = { SIDE ? rows::s2p : rows::s1p, SIDE ? rows::s1p : rows::s2p,
+                                       SIDE ? rows::s3m : rows::s3p, SIDE ? rows::s4m : rows::s4p };
    const double *Recon  = SIDE ? c_st.ReconM[t] : c_st.ReconP[t];
+   if (STD) t = Std_node;
    double s[4];
    if (SIDE) smooth_minus(v, s); else smooth_plus(v, s);
    const double eps = (OP == 0) ? ten_pow_chain(-10) : (OP == 1) ? ten_pow_chain(-8) : ten_pow_chain(-16);
@@ -138,7 +157,7 @@ __device__ __forceinline__ double teno_reconstruct(const double *v, int t, uint3
    double var[4], w[4];
 #pragma unroll
    for (int k = 0; k < 4; k++) {
-      const double *R = &Recon[6*k];
+      const double *R = STD ? std_rows[k] : &Recon[6*k];
       var[k] = v[0]*R[0] + v[1]*R[1] + v[2]*R[2] + v[3]*R[3] + v[4]*R[4] + v[5]*R[5] - base;
       w[k] = Coeffs[k]*b[k];
    }

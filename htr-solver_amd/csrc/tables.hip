@@ -19,12 +19,7 @@ static StencilTables build_tables()
    for (int n = 0; n < 13; n++) for (int k = 0; k < 5; k++) t.Cp[n][k] = cp[n][k];
 
    // the four candidate stencils over [c-2 .. c+3]
-   const double s1p[6] = { 0.0, -1.0/6.0, 5.0/6.0, 2.0/6.0, 0.0, 0.0 };
-   const double s2p[6] = { 0.0, 0.0, 2.0/6.0, 5.0/6.0, -1.0/6.0, 0.0 };
-   const double s3p[6] = { 2.0/6.0, -7.0/6.0, 11.0/6.0, 0.0, 0.0, 0.0 };
-   const double s4p[6] = { 0.0, 0.0, 3.0/12.0, 13.0/12.0, -5.0/12.0, 1.0/12.0 };
-   const double s3m[6] = { 0.0, 0.0, 0.0, 11.0/6.0, -7.0/6.0, 2.0/6.0 };
-   const double s4m[6] = { 1.0/12.0, -5.0/12.0, 13.0/12.0, 3.0/12.0, 0.0, 0.0 };
+   using rows::s1p; using rows::s2p; using rows::s3p; using rows::s4p; using rows::s3m; using rows::s4m;
    const double zr[6]  = { 0, 0, 0, 0, 0, 0 };
    const double e2[6]  = { 0, 0, 1.0, 0, 0, 0 };
    const double e3[6]  = { 0, 0, 0, 1.0, 0, 0 };
@@ -46,7 +41,7 @@ static StencilTables build_tables()
       for (int s = 0; s < 4; s++)
          for (int k = 0; k < 6; k++) { t.ReconP[n][6*s+k] = P[n][s][k]; t.ReconM[n][6*s+k] = M[n][s][k]; }
 
-   const double std4[4] = { 9.0/20.0, 6.0/20.0, 1.0/20.0, 4.0/20.0 };
+   using rows::std4;
    const double c_lp1p[4] = { 2.0/4.0, 1.0/4.0, 0.0, 1.0/4.0 };
    const double c_lp1m[4] = { 7.0/16.0, 8.0/16.0, 1.0/16.0, 0.0 };
    const double c_lp2p[4] = { 47.0/100.0, 27.0/100.0, 10.0/100.0, 16.0/100.0 };
