@@ -50,13 +50,16 @@ RHS_BYTES_PER_CELL = 495.0
 #                   read, face flux 72 B written = 329 B
 #   euler_divergence per cell: face flux 72 B + m_e 8 B read, Conserved_t
 #                   read-modify-write 144 B = 224 B
-#   diffusion       per cell: Conserved[0:5], T, X 5, rho, mu, lam, Di 5,
-#                   velocity 3, 4 metrics (25 doubles) + 3 node types read,
-#                   Conserved_t read-modify-write 144 B = 356 B
+#   diffusion_species per cell: rho Y 5, T, X 5, rho, lambda, D 5, m_s, m_d
+#                   (20 doubles) + nType read, heat flux per face written,
+#                   Conserved_t[0:5] read-modify-write (80 B) = 252 B
+#   diffusion_stress per cell: mu, u 3, m_s, m_d and the transverse m_d 2
+#                   (8 doubles) + 3 node types + heat flux read,
+#                   Conserved_t[5:9] RMW (64 B) = 148 B
 #   chem_source     per cell: rho, p, T, Y 5 (64 B) + Conserved_t[0:5] RMW
 #                   (80 B) = 144 B
 KERNEL_BYTES = {"euler_classify": 5.0, "euler_kg_species": 149.0, "euler_kg_flow": 125.0, "euler_roe": 272.0, "euler_char": 29.0, "euler_finish": 329.0,
-                "euler_divergence": 224.0, "diffusion": 356.0, "chem_source_kernel": 144.0}
+                "euler_divergence": 224.0, "diffusion_species": 252.0, "diffusion_stress": 148.0, "chem_source_kernel": 144.0}
 # FP64 operations per unit of the compute-bound kernels (hand count of the
 # source: split fluxes of six points 528, two TENO reconstructions ~300, the
 # projection constants ~20)

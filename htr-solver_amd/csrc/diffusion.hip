@@ -47,8 +47,10 @@ struct Diffusion {
       for (int i = 0; i < 3; i++) { g1[0][i] = g1[1][i]; g2[0][i] = g2[1][i]; }
    }
 
-   // GetDiffusionFlux (rhs.inl:1140-1218) at the face between p and q
-   __device__ void flux_at(double *Flux, const P3 &p, const P3 &q, const double g1[2][3], const double g2[2][3]) const
+   // GetDiffusionFlux (rhs.inl:1140-1218) at the face between p and q,
+   // species part: the species fluxes into Flux[0..NS-1] and the heat flux
+   // (conduction + species enthalpy transport) returned
+   __device__ double species_at(double *Flux, const P3 &p, const P3 &q) const
    {
       const int t = ld<int>(f.nType[DIR], p);
       const double m_s = ld<double>(f.m_s[DIR], p);
@@ -77,7 +79,16 @@ struct Diffusion {
          Flux[i] += corr;
          heatFlux -= corr*hs[i];
       }
-      // GetSigma (rhs.inl:1058-1137)
+      return heatFlux;
+   }
+
+   // GetDiffusionFlux, momentum part: the viscous stress into Flux[0..2]
+   // and u.sigma returned (GetSigma, rhs.inl:1058-1137)
+   __device__ double stress_at(double *Flux, const P3 &p, const P3 &q, const double g1[2][3],
+                               const double g2[2][3]) const
+   {
+      const int t = ld<int>(f.nType[DIR], p);
+      const double m_s = ld<double>(f.m_s[DIR], p);
       const double mu_s = interp2(t, ld<double>(f.mu, p), ld<double>(f.mu, q));
       double vp[3], vq[3], dn[3], s[3];
 #pragma unroll
@@ -107,11 +118,20 @@ struct Diffusion {
          s[1] = mu_s*(dn[1] + dWdY);
          s[2] = mu_s*(4*dn[2] - 2*dUdX - 2*dVdY)/3;
       }
+
 #pragma unroll
-      for (int i = 0; i < 3; i++) Flux[NS+i] = s[i];
+      for (int i = 0; i < 3; i++) Flux[i] = s[i];
       double uSigma = 0.0;
 #pragma unroll
       for (int i = 0; i < 3; i++) uSigma += interp2(t, vp[i], vq[i])*s[i];
+      return uSigma;
+   }
+
+   // GetDiffusionFlux (rhs.inl:1140-1218) at the face between p and q
+   __device__ void flux_at(double *Flux, const P3 &p, const P3 &q, const double g1[2][3], const double g2[2][3]) const
+   {
+      const double heatFlux = species_at(Flux, p, q);
+      const double uSigma = stress_at(Flux + NS, p, q, g1, g2);
       Flux[NS+3] = (uSigma - heatFlux);
    }
 };
@@ -127,14 +147,40 @@ struct Diffusion {
 // from p0 only when offM1(nType[p0]) = 0, a staggered wall node).  Face j's
 // flux is differenced with face j-1's into Conserved_t of the cell between
 // them with m_d (rhs.inl:1264-1279).
-template<class MX, int DIR>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int L, uint32_t nseg,
-                      uint32_t nunits)
+// Two component groups, each walked by its own launch: G = 0 the species
+// fluxes, whose heat flux (conduction + species enthalpy transport) goes to
+// a face buffer H; G = 1 the viscous stress and the energy flux
+// u.sigma - H, read back per face.  Every value is the one the combined flux
+// forms; the split only halves the registers a thread holds.
+template<class MX, int G>
+struct DiffGroup {
+   static constexpr int LO = (G == 0) ? 0 : MX::NS;   // first conserved component
+   static constexpr int NC = (G == 0) ? MX::NS : 4;    // components of the group
+};
+
+template<class MX, int DIR, int G>
+__device__ __forceinline__ void diff_face(const Diffusion<MX, DIR> &D, double *Fc, const P3 &p, const P3 &q,
+                                          const double g1[2][3], const double g2[2][3], double *__restrict__ H,
+                                          uint32_t fid, bool owner)
+{
+   if constexpr (G == 0) {
+      const double heat = D.species_at(Fc, p, q);
+      if (owner) H[fid] = heat;
+   } else {
+      const double uSigma = D.stress_at(Fc, p, q, g1, g2);
+      Fc[3] = (uSigma - H[fid]);
+   }
+}
+
+template<class MX, int DIR, int G>
+__global__ void __launch_bounds__(256)
+diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ H, int L,
+                      uint32_t nseg, uint32_t nunits)
 {
    const uint32_t u = blockIdx.x*blockDim.x + threadIdx.x;
    if (u >= nunits) return;
-   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B, NEQ = MX::NEQ;
+   constexpr int A = Faces<DIR>::A, B = Faces<DIR>::B;
+   using K = DiffGroup<MX, G>;
    uint32_t line, seg;
    if (DIR == 0) { line = u/nseg; seg = u - line*nseg; }
    else {
@@ -158,17 +204,31 @@ diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    }
    using D_t = Diffusion<MX, DIR>;
    D_t D{f, *mix, g};
-   double g1[2][3], g2[2][3], Fm[NEQ], Fc[NEQ];
-   D.template deriv3<D_t::D1>(gl, g1[0]);
-   D.template deriv3<D_t::D2>(gl, g2[0]);
-   D.flux(Fm, cell, g1, g2);
+   double g1[2][3], g2[2][3], Fm[K::NC], Fc[K::NC];
+   auto face = [&](const P3 &p, int jj, double *F) {
+      const P3 q = wplus(g, DIR, p, 1);
+      if constexpr (G == 1) {
+         D.template deriv3<D_t::D1>(q, g1[1]);
+         D.template deriv3<D_t::D2>(q, g2[1]);
+      }
+      diff_face<MX, DIR, G>(D, F, p, q, g1, g2, H, Faces<DIR>::id(g, line, jj), jj >= j0);
+      if constexpr (G == 1) {
+#pragma unroll
+         for (int i = 0; i < 3; i++) { g1[0][i] = g1[1][i]; g2[0][i] = g2[1][i]; }
+      }
+   };
+   if constexpr (G == 1) {
+      D.template deriv3<D_t::D1>(gl, g1[0]);
+      D.template deriv3<D_t::D2>(gl, g2[0]);
+   }
+   face(cell, j, Fm);
    for (j++; j < j1; j++) {
       cell = (j == 1) ? p0 : wplus(g, DIR, cell, 1);
-      D.flux(Fc, cell, g1, g2);
+      face(cell, j, Fc);
       const double m = ld<double>(f.m_d[DIR], cell);
 #pragma unroll
-      for (int i = 0; i < NEQ; i++) {
-         at<double>(f.Conserved_t, cell, i) += (Fc[i] - Fm[i])*m;
+      for (int i = 0; i < K::NC; i++) {
+         at<double>(f.Conserved_t, cell, K::LO + i) += (Fc[i] - Fm[i])*m;
          Fm[i] = Fc[i];
       }
    }
@@ -181,12 +241,13 @@ diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
 // left gradients at pm1 itself (rhs.inl:1253-1254).
 constexpr int XD_OUT = 62;
 
-template<class MX>
+template<class MX, int G>
 __global__ void __launch_bounds__(256)
-diffusion_x_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, uint32_t nchunk, uint32_t nwaves)
+diffusion_x_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ H,
+                   uint32_t nchunk, uint32_t nwaves)
 {
-   constexpr int NEQ = MX::NEQ;
    using D_t = Diffusion<MX, 0>;
+   using K = DiffGroup<MX, G>;
    const uint32_t wave = (blockIdx.x*blockDim.x + threadIdx.x) >> 6;
    if (wave >= nwaves) return;                                      // whole waves
    const int lane = threadIdx.x & 63;
@@ -201,31 +262,35 @@ diffusion_x_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
    const P3 q = wplus(g, 0, p, 1);
    D_t D{f, *mix, g};
    double g1[2][3], g2[2][3];
+   if constexpr (G == 1) {
 #pragma unroll
-   for (int i = 0; i < 3; i++) { g1[1][i] = 0.0; g2[1][i] = 0.0; }
-   if (live) {
-      D.template deriv3<D_t::D1>(q, g1[1]);
-      D.template deriv3<D_t::D2>(q, g2[1]);
-   }
-#pragma unroll
-   for (int i = 0; i < 3; i++) { g1[0][i] = __shfl_up(g1[1][i], 1, 64); g2[0][i] = __shfl_up(g2[1][i], 1, 64); }
-   double Fc[NEQ];
-#pragma unroll
-   for (int i = 0; i < NEQ; i++) Fc[i] = 0.0;
-   if (lane >= 1 && live && j >= 0) {
-      if (j == 0) {
-         D.template deriv3<D_t::D1>(p, g1[0]);
-         D.template deriv3<D_t::D2>(p, g2[0]);
+      for (int i = 0; i < 3; i++) { g1[1][i] = 0.0; g2[1][i] = 0.0; }
+      if (live) {
+         D.template deriv3<D_t::D1>(q, g1[1]);
+         D.template deriv3<D_t::D2>(q, g2[1]);
       }
-      D.flux_at(Fc, p, q, g1, g2);
-   }
-   double Fm[NEQ];
 #pragma unroll
-   for (int i = 0; i < NEQ; i++) Fm[i] = __shfl_up(Fc[i], 1, 64);
+      for (int i = 0; i < 3; i++) { g1[0][i] = __shfl_up(g1[1][i], 1, 64); g2[0][i] = __shfl_up(g2[1][i], 1, 64); }
+   }
+   double Fc[K::NC];
+#pragma unroll
+   for (int i = 0; i < K::NC; i++) Fc[i] = 0.0;
+   if (lane >= 1 && live && j >= 0) {
+      if constexpr (G == 1) {
+         if (j == 0) {
+            D.template deriv3<D_t::D1>(p, g1[0]);
+            D.template deriv3<D_t::D2>(p, g2[0]);
+         }
+      }
+      diff_face<MX, 0, G>(D, Fc, p, q, g1, g2, H, row*(uint32_t)(esz + 1) + (uint32_t)j, lane >= 2);
+   }
+   double Fm[K::NC];
+#pragma unroll
+   for (int i = 0; i < K::NC; i++) Fm[i] = __shfl_up(Fc[i], 1, 64);
    if (lane >= 2 && live && j >= 1) {
       const double m = ld<double>(f.m_d[0], p);
 #pragma unroll
-      for (int i = 0; i < NEQ; i++) at<double>(f.Conserved_t, p, i) += (Fc[i] - Fm[i])*m;
+      for (int i = 0; i < K::NC; i++) at<double>(f.Conserved_t, p, K::LO + i) += (Fc[i] - Fm[i])*m;
    }
 }
 
@@ -234,20 +299,33 @@ static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipSt
 {
    const int64_t ncells = exec_cells(g);
    if (ncells == 0) return HTR_OK;
+   const int64_t nfaces = Faces<DIR>::count(g);
+   if (nfaces >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many faces");
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   int rc = workspace_faces(*wp, nfaces, 0, 0);
+   if (rc) return rc;
+   double *H = wp->faces;                          // heat flux per face
    if constexpr (DIR == 0) {
       const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XD_OUT - 1)/XD_OUT;
       const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
       if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many rows");
-      HTR_LAUNCH("diffusion_x", (diffusion_x_kernel<MX>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0, s, f, mix, g,
-                 (uint32_t)nchunk, (uint32_t)nwaves);
+      HTR_LAUNCH("diffusion_species_x", (diffusion_x_kernel<MX, 0>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0, s,
+                 f, mix, g, H, (uint32_t)nchunk, (uint32_t)nwaves);
+      if ((rc = check_launch("diffusion_x_kernel"))) return rc;
+      HTR_LAUNCH("diffusion_stress_x", (diffusion_x_kernel<MX, 1>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0, s,
+                 f, mix, g, H, (uint32_t)nchunk, (uint32_t)nwaves);
       return check_launch("diffusion_x_kernel");
    } else {
       const int L = walk_segment(DIR);
       const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
       const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
       if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many line segments");
-      HTR_LAUNCH(HTR_DIR_NAME("diffusion", DIR), (diffusion_walk_kernel<MX, DIR>), dim3(grid_for(nunits, 256)),
-                 dim3(256), 0, s, f, mix, g, L, (uint32_t)nseg, (uint32_t)nunits);
+      HTR_LAUNCH(HTR_DIR_NAME("diffusion_species", DIR), (diffusion_walk_kernel<MX, DIR, 0>),
+                 dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, mix, g, H, L, (uint32_t)nseg, (uint32_t)nunits);
+      if ((rc = check_launch("diffusion_walk_kernel"))) return rc;
+      HTR_LAUNCH(HTR_DIR_NAME("diffusion_stress", DIR), (diffusion_walk_kernel<MX, DIR, 1>),
+                 dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, mix, g, H, L, (uint32_t)nseg, (uint32_t)nunits);
       return check_launch("diffusion_walk_kernel");
    }
 }
