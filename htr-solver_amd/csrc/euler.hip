@@ -303,11 +303,12 @@ struct Euler {
       put_rec(rec, cap, w, R, Lam1, Lam, LamN);
    }
 
-   // pass C: back-projection (rhs.inl:708) and the positivity limiter
-   // (rhs.inl:710-758); returns the decision word
+   // pass C: back-projection (rhs.inl:708) and the positivity test of the
+   // limiter (rhs.inl:710-720); returns the decision word without the
+   // limiter bit, *limit says whether the face needs the limiter blend
    __device__ uint32_t face_finish(double *Flux, const double *__restrict__ rec, int64_t cap, int64_t w,
                                    const double *__restrict__ Fc, const uint8_t *__restrict__ bits,
-                                   const P3 &p, int t, double RK0, double RK1, double dt) const
+                                   const P3 &p, int t, double RK0, double RK1, double dt, bool *limit) const
    {
       Roe R;
       get_rec(rec, cap, w, R);
@@ -321,63 +322,72 @@ struct Euler {
          const uint32_t b = bits[i*cap + w];
          h = fold_bits(fold_bits(h, b & 15u), b >> 4);
       }
-      uint32_t lim_bit = 0;
+      *limit = false;
       if (NS > 1) {
          const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
          const double lim_f = dt*(ld<double>(f.m_e[DIR], p) + ld<double>(f.m_e[DIR], pP1));
-         bool limit = false;
 #pragma unroll
          for (int i = 0; i < NS; i++) {
             const double dRhoi = lim_f*Flux[i];
             if (((RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i) + dRhoi) < -1e-8) ||
                 ((RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*cons(pP1, i) - dRhoi) < -1e-8))
-               limit = true;
-         }
-         if (limit) {
-            lim_bit = 2;
-            // first-order flux -0.5*(FluxP(p) + FluxM(p+1)), back-projected
-            Proj J;
-            proj(J, R);
-            double FLO[NEQ];
-            double sq = 0.0, s2[4], s3[4];
-            point_sums(p, R, J, s2);
-            point_sums(pP1, R, J, s3);
-#pragma unroll
-            for (int i = 0; i < NEQ; i++) {
-               const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
-               const double Ysel = (i >= 1 && i <= NS) ? R.Y[i-1] : 0.0;
-               double ap, am, bp, bm;
-               split(ap, am, i, p, lam, R, J, Ysel, s2);
-               split(bp, bm, i, pP1, lam, R, J, Ysel, s3);
-               add_column(FLO, sq, i, -0.5*(ap + bm), R, ERoe, dedp);
-            }
-            const double eps = 1e-60;
-            double theta = 1.0;
-#pragma unroll
-            for (int i = 0; i < NS; i++) {
-               const double dRhoi = lim_f*Flux[i];
-               const double dRhoi_lo = lim_f*FLO[i];
-               double theta_p = 1.0;
-               const double rhoi_p = RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i);
-               const double rhoHO_p = rhoi_p + dRhoi;
-               if (rhoHO_p < eps) {
-                  const double rhoLO = rhoi_p + dRhoi_lo;
-                  theta_p = (fabs(rhoHO_p - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_p - rhoLO)) : 1.0;
-               }
-               double theta_pp1 = 1.0;
-               const double rhoi_pp1 = RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*cons(pP1, i);
-               const double rhoHO_pp1 = rhoi_pp1 - dRhoi;
-               if (rhoHO_pp1 < eps) {
-                  const double rhoLO = rhoi_pp1 - dRhoi_lo;
-                  theta_pp1 = (fabs(rhoHO_pp1 - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_pp1 - rhoLO)) : 1.0;
-               }
-               theta = dmin(dmin(theta_p, theta_pp1), theta);
-            }
-#pragma unroll
-            for (int i = 0; i < NEQ; i++) Flux[i] = Flux[i]*theta + FLO[i]*(1 - theta);
+               *limit = true;
          }
       }
-      return (h << 2) | lim_bit;
+      return h << 2;
+   }
+
+   // pass D, on the faces the test flagged: the first-order flux
+   // -0.5*(FluxP(p) + FluxM(p+1)) back-projected and blended with the
+   // high-order Flux (rhs.inl:721-758)
+   __device__ void face_limit(double *Flux, const double *__restrict__ rec, int64_t cap, int64_t w, const P3 &p,
+                              int t, double RK0, double RK1, double dt) const
+   {
+      Roe R;
+      get_rec(rec, cap, w, R);
+      const double dedp = 1.0/R.dpde;
+      const double ERoe = R.e + 0.5*mod2(R.v);
+      const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
+      const double lim_f = dt*(ld<double>(f.m_e[DIR], p) + ld<double>(f.m_e[DIR], pP1));
+      Proj J;
+      proj(J, R);
+      double FLO[NEQ];
+      double sq = 0.0, s2[4], s3[4];
+      point_sums(p, R, J, s2);
+      point_sums(pP1, R, J, s3);
+#pragma unroll
+      for (int i = 0; i < NEQ; i++) {
+         const double lam = rec[((i == 0) ? 2*NS+9 : (i == NEQ-1) ? 2*NS+11 : 2*NS+10)*cap + w];
+         const double Ysel = (i >= 1 && i <= NS) ? R.Y[i-1] : 0.0;
+         double ap, am, bp, bm;
+         split(ap, am, i, p, lam, R, J, Ysel, s2);
+         split(bp, bm, i, pP1, lam, R, J, Ysel, s3);
+         add_column(FLO, sq, i, -0.5*(ap + bm), R, ERoe, dedp);
+      }
+      const double eps = 1e-60;
+      double theta = 1.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+         const double dRhoi = lim_f*Flux[i];
+         const double dRhoi_lo = lim_f*FLO[i];
+         double theta_p = 1.0;
+         const double rhoi_p = RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i);
+         const double rhoHO_p = rhoi_p + dRhoi;
+         if (rhoHO_p < eps) {
+            const double rhoLO = rhoi_p + dRhoi_lo;
+            theta_p = (fabs(rhoHO_p - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_p - rhoLO)) : 1.0;
+         }
+         double theta_pp1 = 1.0;
+         const double rhoi_pp1 = RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*cons(pP1, i);
+         const double rhoHO_pp1 = rhoi_pp1 - dRhoi;
+         if (rhoHO_pp1 < eps) {
+            const double rhoLO = rhoi_pp1 - dRhoi_lo;
+            theta_pp1 = (fabs(rhoHO_pp1 - rhoLO) > eps) ? dmax(0.0, (eps - rhoLO)/(rhoHO_pp1 - rhoLO)) : 1.0;
+         }
+         theta = dmin(dmin(theta_p, theta_pp1), theta);
+      }
+#pragma unroll
+      for (int i = 0; i < NEQ; i++) Flux[i] = Flux[i]*theta + FLO[i]*(1 - theta);
    }
 
    __device__ bool sensor_kg(const P3 &p, int t) const { return hybrid_kg<DIR>(f, g, p, t); }
@@ -1057,24 +1067,64 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    }
 }
 
+// Faces whose positivity test fails are appended to the limiter list llist
+// (count in lcount); the blend runs in euler_limit_kernel on those alone, so
+// that this pass does not hold the first-order flux's registers.
 template<class MX, int DIR, bool LISTED>
 __global__ void __launch_bounds__(256)
 euler_finish_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv,
                     double RK0, double RK1,
                     double dt, TenoBufs b, double *__restrict__ F, int64_t nfaces,
-                    const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec)
+                    const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec,
+                    int32_t *__restrict__ llist, int *__restrict__ lcount)
 {
    const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
    const int64_t stride = (int64_t)gridDim.x*blockDim.x;
    Euler<MX, DIR> E{f, *mix, g, hv};
-   for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < n; w += stride) {
+   const int64_t nround = (n + stride - 1)/stride*stride;      // whole waves run every round (ballot)
+   for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < nround; w += stride) {
+      bool limit = false;
+      if (w < n) {
+         const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
+         const P3 q = Faces<DIR>::cell(g, f, fid);
+         double Flux[MX::NEQ];
+         const uint32_t word = E.face_finish(Flux, b.rec, b.cap, w, b.Fc, b.bits, q, ld<int>(f.nType[DIR], q),
+                                             RK0, RK1, dt, &limit);
+         store_face<MX::NEQ>(F, nfaces, fid, Flux);
+         if (dec.ptr) at<uint32_t>(dec, q) = word;
+      }
+      const uint64_t mask = __ballot(limit);
+      if (mask) {
+         const int lane = threadIdx.x & 63;
+         const int leader = __ffsll((unsigned long long)mask) - 1;
+         int base = 0;
+         if (lane == leader) base = atomicAdd(lcount, __popcll(mask));
+         base = __shfl(base, leader, 64);
+         if (limit) llist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)w;
+      }
+   }
+}
+
+template<class MX, int DIR, bool LISTED>
+__global__ void __launch_bounds__(256)
+euler_limit_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv,
+                   double RK0, double RK1, double dt, TenoBufs b, double *__restrict__ F, int64_t nfaces,
+                   const int32_t *__restrict__ list, const int32_t *__restrict__ llist,
+                   const int *__restrict__ lcount, htr_view dec)
+{
+   const int64_t n = *lcount;
+   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
+   Euler<MX, DIR> E{f, *mix, g, hv};
+   for (int64_t k = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; k < n; k += stride) {
+      const int64_t w = llist[k];
       const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
       const P3 q = Faces<DIR>::cell(g, f, fid);
       double Flux[MX::NEQ];
-      const uint32_t word = E.face_finish(Flux, b.rec, b.cap, w, b.Fc, b.bits, q, ld<int>(f.nType[DIR], q),
-                                          RK0, RK1, dt);
+#pragma unroll
+      for (int i = 0; i < MX::NEQ; i++) Flux[i] = F[i*nfaces + fid];
+      E.face_limit(Flux, b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q), RK0, RK1, dt);
       store_face<MX::NEQ>(F, nfaces, fid, Flux);
-      if (dec.ptr) at<uint32_t>(dec, q) = word;
+      if (dec.ptr) at<uint32_t>(dec, q) |= 2u;
    }
 }
 
@@ -1114,9 +1164,14 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
    HTR_LAUNCH(HTR_DIR_NAME("euler_char", DIR), (euler_char_lds_kernel<MX, DIR, SCHEME, LISTED>), dim3(nchunk),
               dim3(64*MX::NEQ), 0, s, f, mix, g, hv, b, nfaces, w.list, w.counter);
    rc = check_launch("euler_char_kernel"); if (rc) return rc;
+   int32_t *llist = w.list + nfaces;                  // limiter list after the TENO list
    HTR_LAUNCH(HTR_DIR_NAME("euler_finish", DIR), (euler_finish_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
-                      f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec);
-   return check_launch("euler_finish_kernel");
+                      f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec, llist, w.counter + 2);
+   if ((rc = check_launch("euler_finish_kernel"))) return rc;
+   if (MX::NS == 1) return HTR_OK;                   // single species: no positivity limiter
+   HTR_LAUNCH(HTR_DIR_NAME("euler_limit", DIR), (euler_limit_kernel<MX, DIR, LISTED>), dim3(256), dim3(256), 0, s,
+              f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, llist, w.counter + 2, dec);
+   return check_launch("euler_limit_kernel");
 }
 
 template<class MX, int DIR, int SCHEME>
@@ -1130,7 +1185,7 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    Workspace *wp = workspace(s);
    if (!wp) return HTR_E_DEVICE;
    Workspace &w = *wp;
-   int rc = workspace_faces(w, SCHEME == HTR_EULER_KG ? 0 : nfaces*MX::NEQ, SCHEME == HTR_EULER_HYBRID ? nfaces : 0,
+   int rc = workspace_faces(w, SCHEME == HTR_EULER_KG ? 0 : nfaces*MX::NEQ, SCHEME == HTR_EULER_KG ? 0 : 2*nfaces,
                             SCHEME == HTR_EULER_KG ? 0 : teno_doubles<MX>(nfaces));
    if (rc) return rc;
    // total enthalpy of every cell of the bounds
@@ -1143,8 +1198,10 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    hv.sx = 8;                                  // one double per base element (k = 1)
    HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
    if ((rc = check_launch("enthalpy_kernel"))) return rc;
+   if constexpr (SCHEME != HTR_EULER_KG) {
+      if ((rc = zero_counters(w.counter, s))) return rc;    // TENO list and limiter list counts
+   }
    if constexpr (SCHEME == HTR_EULER_HYBRID) {
-      if ((rc = zero_counters(w.counter, s))) return rc;
       HTR_LAUNCH(HTR_DIR_NAME("euler_classify", DIR), (euler_classify_kernel<MX, DIR>),
                  dim3(grid_for(nfaces, 256*CLS_PER_THREAD)),
                  dim3(256), 0, s, f, g, nfaces, w.list, w.counter + 1);
