@@ -458,10 +458,43 @@ __device__ inline void store_face(double *__restrict__ F, int64_t nfaces, uint32
 }
 
 // Pass 1 of the hybrid scheme: the hybrid sensor of every face; faces that
-// need TENO are appended to a compacted list.  A block of 256 threads takes
-// CLS_PER_THREAD*256 consecutive faces and reserves its list range with one
-// atomic (one per wave would serialise some 10^5 atomics on one address).
+// need TENO are appended to a compacted list.  A block takes 4096 faces and
+// reserves its list range with one atomic (one per wave would serialise some
+// 10^5 atomics on one address).  The TENO passes gather the six stencil
+// points of every listed face, so the list keeps faces that share stencil
+// rows together: for the y and z sweeps a block's faces are a tile of 64
+// x-cells x 8 faces along the sweep x 8 cells of the remaining direction
+// (a wave takes one x-row of the tile); for the x sweep, whose stencil runs
+// along the row, 4096 consecutive face ids.
 constexpr int CLS_PER_THREAD = 16;
+
+template<int DIR>
+struct ClsTiles {
+   static constexpr int B = Faces<DIR>::B;
+   __host__ __device__ static uint32_t ntx(const Geo &g) { return ((uint32_t)g.esz[0] + 63u)/64u; }
+   __host__ __device__ static uint32_t ntj(const Geo &g) { return ((uint32_t)g.esz[DIR] + 8u)/8u; }
+   __host__ __device__ static uint32_t ntb(const Geo &g) { return ((uint32_t)g.esz[B] + 7u)/8u; }
+   static int64_t blocks(const Geo &g, int64_t nfaces)
+   {
+      if (DIR == 0) return (nfaces + 4095)/4096;
+      return (int64_t)ntx(g)*ntj(g)*ntb(g);
+   }
+   // face id of element e (0..4095) of block b; false past the rect
+   __device__ static bool face(const Geo &g, int64_t nfaces, uint32_t b, uint32_t e, uint32_t &fid)
+   {
+      if (DIR == 0) {
+         const int64_t id = (int64_t)b*4096 + e;
+         fid = (uint32_t)id;
+         return id < nfaces;
+      }
+      const uint32_t nx = ntx(g), nj = ntj(g);
+      const uint32_t tx = b % nx, r = b/nx, tj = r % nj, tb = r/nj;
+      const uint32_t x = tx*64u + (e & 63u), j = tj*8u + ((e >> 6) & 7u), c = tb*8u + (e >> 9);
+      if (x >= (uint32_t)g.esz[0] || j > (uint32_t)g.esz[DIR] || c >= (uint32_t)g.esz[B]) return false;
+      fid = j*((uint32_t)g.esz[0]*(uint32_t)g.esz[B]) + x + c*(uint32_t)g.esz[0];
+      return true;
+   }
+};
 
 template<class MX, int DIR>
 __global__ void __launch_bounds__(256)
@@ -471,16 +504,15 @@ euler_classify_kernel(const htr_fields f, const Geo g, int64_t nfaces, int32_t *
    __shared__ int wave_count[4];
    __shared__ int block_base;
    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-   const int64_t tile = (int64_t)blockIdx.x*(256*CLS_PER_THREAD);
    uint32_t flags = 0;
    int before = 0;                    // TENO faces of this wave in earlier rounds
    uint64_t masks[CLS_PER_THREAD];
 #pragma unroll
    for (int k = 0; k < CLS_PER_THREAD; k++) {
-      const int64_t fid = tile + k*256 + threadIdx.x;
+      uint32_t fid;
       bool teno = false;
-      if (fid < nfaces) {
-         const P3 q = Faces<DIR>::cell(g, f, (uint32_t)fid);
+      if (ClsTiles<DIR>::face(g, nfaces, blockIdx.x, k*256 + threadIdx.x, fid)) {
+         const P3 q = Faces<DIR>::cell(g, f, fid);
          teno = !hybrid_kg<DIR>(f, g, q, ld<int>(f.nType[DIR], q));
       }
       masks[k] = __ballot(teno);
@@ -499,7 +531,11 @@ euler_classify_kernel(const htr_fields f, const Geo g, int64_t nfaces, int32_t *
    const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
    for (int k = 0; k < CLS_PER_THREAD; k++) {
-      if (flags & (1u << k)) list[base + __popcll(masks[k] & lt)] = (int32_t)(tile + k*256 + threadIdx.x);
+      if (flags & (1u << k)) {
+         uint32_t fid = 0;
+         ClsTiles<DIR>::face(g, nfaces, blockIdx.x, k*256 + threadIdx.x, fid);
+         list[base + __popcll(masks[k] & lt)] = (int32_t)fid;
+      }
       base += __popcll(masks[k]);
    }
 }
@@ -1203,7 +1239,7 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    }
    if constexpr (SCHEME == HTR_EULER_HYBRID) {
       HTR_LAUNCH(HTR_DIR_NAME("euler_classify", DIR), (euler_classify_kernel<MX, DIR>),
-                 dim3(grid_for(nfaces, 256*CLS_PER_THREAD)),
+                 dim3((unsigned)ClsTiles<DIR>::blocks(g, nfaces)),
                  dim3(256), 0, s, f, g, nfaces, w.list, w.counter + 1);
       if ((rc = check_launch("euler_classify_kernel"))) return rc;
       if ((rc = launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
