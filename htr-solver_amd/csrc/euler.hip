@@ -450,11 +450,21 @@ struct Faces {
    __host__ __device__ static int64_t count(const Geo &g) { return ((int64_t)g.esz[DIR] + 1)*g.esz[A]*g.esz[B]; }
 };
 
+// Face buffer, face-major: the nEq components of face fid are F[fid*nEq ..
+// fid*nEq + nEq - 1], so the scattered TENO faces write and read one
+// cache-line run each instead of nEq separate lines.
+template<int NEQ>
+__device__ inline double &face_at(double *__restrict__ F, uint32_t fid, int i) { return F[(int64_t)fid*NEQ + i]; }
+
+template<int NEQ>
+__device__ inline double face_ld(const double *__restrict__ F, uint32_t fid, int i) { return F[(int64_t)fid*NEQ + i]; }
+
 template<int NEQ>
 __device__ inline void store_face(double *__restrict__ F, int64_t nfaces, uint32_t fid, const double *v)
 {
+   (void)nfaces;
 #pragma unroll
-   for (int i = 0; i < NEQ; i++) F[i*nfaces + fid] = v[i];
+   for (int i = 0; i < NEQ; i++) face_at<NEQ>(F, fid, i) = v[i];
 }
 
 // Pass 1 of the hybrid scheme: the hybrid sensor of every face; faces that
@@ -688,7 +698,7 @@ __device__ void kg_walk(const htr_fields &f, const Geo &g, const htr_view &hv, c
       if (!kg) {
          const uint32_t fid = Faces<DIR>::id(g, line, j);
 #pragma unroll
-         for (int i = 0; i < K::NC; i++) Fl[i] = F[(int64_t)(K::LO + i)*nfaces + fid];
+         for (int i = 0; i < K::NC; i++) Fl[i] = face_ld<MX::NEQ>(F, fid, K::LO + i);
       } else if (G == 0 && dec.ptr) {
          at<uint32_t>(dec, p) = 1;
       }
@@ -810,7 +820,7 @@ euler_kg_x_kernel(const htr_fields f, const Geo g, const htr_view hv, const doub
       if (!kg) {
          const uint32_t fid = row*(uint32_t)(esz + 1) + (uint32_t)j;
 #pragma unroll
-         for (int i = 0; i < K::NC; i++) Fc[i] = F[(int64_t)(K::LO + i)*nfaces + fid];
+         for (int i = 0; i < K::NC; i++) Fc[i] = face_ld<MX::NEQ>(F, fid, K::LO + i);
       } else if (q.o != p.o) {
          KGRing<MX, 0, G> ring;
          ring.warm(f, hv, g, q);
@@ -1157,7 +1167,7 @@ euler_limit_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
       const P3 q = Faces<DIR>::cell(g, f, fid);
       double Flux[MX::NEQ];
 #pragma unroll
-      for (int i = 0; i < MX::NEQ; i++) Flux[i] = F[i*nfaces + fid];
+      for (int i = 0; i < MX::NEQ; i++) Flux[i] = face_ld<MX::NEQ>(F, fid, i);
       E.face_limit(Flux, b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q), RK0, RK1, dt);
       store_face<MX::NEQ>(F, nfaces, fid, Flux);
       if (dec.ptr) at<uint32_t>(dec, q) |= 2u;
@@ -1182,7 +1192,7 @@ face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict
    const double m = ld<double>(metric, p);
 #pragma unroll
    for (int c = 0; c < NEQ; c++)
-      at<double>(f.Conserved_t, p, c) += (F[c*nfaces + fp] - F[c*nfaces + fm])*m;
+      at<double>(f.Conserved_t, p, c) += (face_ld<NEQ>(F, fp, c) - face_ld<NEQ>(F, fm, c))*m;
 }
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
