@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
+    ap.add_argument("--layout", default="soa", choices=["soa", "aos"],
+                    help="vector fields structure-of-arrays, or per point as the reference's VecNEq/VecNSp/Vec3")
     ap.add_argument("--workload", default="air", choices=["air", "bl"],
                     help="air: 256^3 periodic air box; bl: 512x256x128 Mach 6 boundary layer")
     ap.add_argument("--chem-n", type=int, default=0,
@@ -120,14 +122,14 @@ def setup_bl(seed):
     return fs, mix, 0.5 / r, meta, grid
 
 
-def setup_gpu(n, seed, composition="per-seed"):
+def setup_gpu(n, seed, composition="per-seed", layout="soa"):
     import numpy as np
     import torch
     import htr_amd as H
     from htr_amd import cases
     st, meta = cases.air_box(n, seed, composition=composition)
     mix = H.Mixture.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
-    fs = H.FieldSet((n, n, n), 5)
+    fs = H.FieldSet((n, n, n), 5, layout=layout)
     for k, v in st.items():
         fs.t[k].copy_(torch.from_numpy(np.ascontiguousarray(v)).to(fs.t[k].dtype))
     del st
@@ -386,7 +388,7 @@ def main():
         cells = 512 * 256 * 128
         nfaces_dir = [(513 * 256 * 128), (512 * 257 * 128), (512 * 256 * 129)]
     else:
-        fs, mix, dt, meta = setup_gpu(n, a.seed, a.composition)
+        fs, mix, dt, meta = setup_gpu(n, a.seed, a.composition, a.layout)
         grid = None
         cells = n ** 3
         nfaces_dir = [(n + 1) * n * n] * 3
@@ -422,7 +424,8 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_unit": bpu, "units_per_launch": units,
             "launch_ms": launch_ms}
-    default_workload = a.workload == "air" and n == 256 and a.composition == "per-seed" and a.scheme == "hybrid"
+    default_workload = (a.workload == "air" and n == 256 and a.composition == "per-seed" and a.scheme == "hybrid"
+                        and a.layout == "soa")
     traffic = measured_traffic(dom) if default_workload else None
     if traffic is not None:
         roof["traffic"] = traffic
@@ -444,7 +447,8 @@ def main():
                    else "air5_bl_512x256x128_rhs_%s" % a.scheme,
                    "grid": [n, n, n] if a.workload == "air" else [512, 256, 128],
                    "species": 5, "scheme": a.scheme, "composition": a.composition if a.workload == "air" else None,
-                   "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed},
+                   "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed,
+                   "layout": a.layout if a.workload == "air" else "soa"},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
         "teno_face_frac": sum(c[0] for c in counts.values()) / faces,
         "limiter_faces": {d: c[1] for d, c in counts.items()},
