@@ -317,7 +317,7 @@ static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipSt
                  f, mix, g, H, (uint32_t)nchunk, (uint32_t)nwaves);
       return check_launch("diffusion_x_kernel");
    } else {
-      const int L = walk_segment(DIR);
+      const int L = walk_segment(DIR, (int64_t)g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B], (int64_t)g.esz[DIR] + 1);
       const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
       const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
       if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many line segments");

@@ -1267,7 +1267,7 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
                     s, f, g, hv, w.faces, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
          return check_launch("euler_kg_x_kernel");
       } else {
-         const int L = walk_segment(DIR);
+         const int L = walk_segment(DIR, (int64_t)g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B], (int64_t)g.esz[DIR] + 1);
          const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
          const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
          if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many line segments");

@@ -70,21 +70,26 @@ void prof_end(hipStream_t s);
 // "name_x", "name_y", "name_z"
 #define HTR_DIR_NAME(base, dir) ((dir) == 0 ? base "_x" : (dir) == 1 ? base "_y" : base "_z")
 
-// Faces per segment of the y and z line walks (KG and diffusion); the x
-// sweeps use wave-wide rows instead and ignore entry 0.  HTR_WALK_SEGMENT=
-// "Lx,Ly,Lz" overrides (tuning).
-static inline int walk_segment(int dir)
+// Faces per segment of the y and z line walks (KG and diffusion): the
+// longest of 64, 32, 16, 8 that still gives at least 2^18 threads (16 waves
+// per CU), since a longer segment repeats less warm-up but a short rect then
+// leaves CUs idle.  HTR_WALK_SEGMENT="Lx,Ly,Lz" fixes it (tuning; entry 0
+// is unused: the x sweeps take wave-wide rows).
+static inline int walk_segment(int dir, int64_t nlines, int64_t faces_per_line)
 {
-   static int seg[3] = { 0, 0, 0 };
-   if (seg[0] == 0) {
-      int v[3] = { 1, 32, 32 };
+   static int fixed[3] = { -1, -1, -1 };
+   if (fixed[0] < 0) {
+      int v[3] = { 0, 0, 0 };
       if (const char *e = getenv("HTR_WALK_SEGMENT")) {
          int a, b, c;
          if (sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && a > 0 && b > 0 && c > 0) { v[0] = a; v[1] = b; v[2] = c; }
       }
-      seg[1] = v[1]; seg[2] = v[2]; seg[0] = v[0];
+      fixed[1] = v[1]; fixed[2] = v[2]; fixed[0] = v[0];
    }
-   return seg[dir];
+   if (fixed[dir] > 0) return fixed[dir];
+   int L = 64;
+   while (L > 8 && nlines*((faces_per_line + L - 1)/L) < (int64_t(1) << 18)) L /= 2;
+   return L;
 }
 
 } // namespace htr
