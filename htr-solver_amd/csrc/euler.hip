@@ -966,7 +966,7 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    // (TENO-A keeps no values across the compute: its reconstruction needs
    // the registers, so it stages each chunk straight into LDS)
    constexpr bool PF = SCHEME != HTR_EULER_TENOA;
-   double pv[PF ? MV : 1][6];
+   double pv[PF ? MV : 1][6], rv[PF ? MR : 1];
    int tv = 0;
    auto fetch = [&](uint32_t c) {
       const uint32_t w = c*64u + (uint32_t)lane;
@@ -978,6 +978,13 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
          t = ld<int>(f.nType[DIR], q);
       }
       tv = t;                                     // Std_node (0) on lanes past the end
+      if constexpr (PF) {
+#pragma unroll
+         for (int m = 0; m < MR; m++) {
+            const int r = wv + m*NEQ;
+            rv[m] = (valid && r < NRV) ? b.rec[(int64_t)char_rec_slot<MX>(r)*b.cap + w] : 0.0;
+         }
+      }
 #pragma unroll
       for (int m = 0; m < MV; m++) {
          const int v = wv + m*NEQ;
@@ -1012,7 +1019,7 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
 #pragma unroll
          for (int m = 0; m < MR; m++) {
             const int r = wv + m*NEQ;
-            if (r < NRV) S.rc[r][lane] = valid ? b.rec[(int64_t)char_rec_slot<MX>(r)*b.cap + w] : 0.0;
+            if (r < NRV) S.rc[r][lane] = rv[m];
          }
       }
       if (wv == 0) S.t[lane] = tv;
