@@ -264,16 +264,29 @@ static MixKind kind_of(const htr_mix *m)
    return K_NONE;
 }
 
+// Positive status inside the library: the execution rect is empty, the call
+// has nothing to do (entry points return HTR_OK without launching).
+constexpr int kEmptyRect = 1;
+
+// An execution rect with hi < lo in any direction is empty (a per-direction
+// rect intersection can produce one); it launches nothing, wherever its
+// corners lie.  A non-empty rect must lie inside the bounds.
 static int check_bounds(const htr_grid *g)
 {
    for (int d = 0; d < 3; d++) {
       if (g->bounds.hi[d] < g->bounds.lo[d]) return set_error(HTR_E_ARG, "empty bounds");
-      if (g->exec.lo[d] < g->bounds.lo[d] || g->exec.hi[d] > g->bounds.hi[d])
-         return set_error(HTR_E_ARG, "execution rect outside bounds");
       if (g->bounds.hi[d] - g->bounds.lo[d] >= (1ll << 30)) return set_error(HTR_E_UNSUPPORTED, "bounds too large");
    }
+   for (int d = 0; d < 3; d++)
+      if (g->exec.hi[d] < g->exec.lo[d]) return kEmptyRect;
+   for (int d = 0; d < 3; d++)
+      if (g->exec.lo[d] < g->bounds.lo[d] || g->exec.hi[d] > g->bounds.hi[d])
+         return set_error(HTR_E_ARG, "execution rect outside bounds");
    return HTR_OK;
 }
+
+// status of an entry point after a prep step: an empty rect is success
+static inline int done(int rc) { return rc < 0 ? rc : HTR_OK; }
 
 static int64_t gcd64(int64_t a, int64_t b) { while (b) { const int64_t t = a % b; a = b; b = t; } return a; }
 
@@ -435,6 +448,30 @@ int htr_profile_report(char *buf, size_t len)
    return HTR_OK;
 }
 
+int htr_stencil_table(int table, int node, int k, double *value)
+{
+   if (!value) return set_error(HTR_E_ARG, "null value");
+   if (node < 0 || node >= N_NODE_TYPES) return set_error(HTR_E_ARG, "bad node type %d", node);
+   const StencilTables &t = host_stencil_tables();
+   static const int len[11] = { 5, 24, 24, 4, 4, 2, 2, 1, 3, 1, 14 };
+   if (table < 0 || table > 10) return set_error(HTR_E_ARG, "bad table %d", table);
+   if (k < 0 || k >= len[table]) return set_error(HTR_E_ARG, "bad entry %d of table %d", k, table);
+   switch (table) {
+      case 0:  *value = t.Cp[node][k]; break;
+      case 1:  *value = t.ReconP[node][k]; break;
+      case 2:  *value = t.ReconM[node][k]; break;
+      case 3:  *value = t.CoeffP[node][k]; break;
+      case 4:  *value = t.CoeffM[node][k]; break;
+      case 5:  *value = t.Interp[node][k]; break;
+      case 6:  *value = t.Grad[node][k]; break;
+      case 7:  *value = t.KOrder[node]; break;
+      case 8:  *value = t.KCoeff[node][k]; break;
+      case 9:  *value = t.KNSum[node]; break;
+      default: *value = t.p10[k]; break;
+   }
+   return HTR_OK;
+}
+
 int htr_init(int device)
 {
    if (hipSetDevice(device) != hipSuccess) return set_error(HTR_E_DEVICE, "hipSetDevice(%d) failed", device);
@@ -553,7 +590,7 @@ void htr_mix_destroy(htr_mix *m)
 
 int htr_update_properties(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    HTR_NEED(pressure); HTR_NEED(temperature); HTR_NEED(MolarFracs); HTR_NEED(MassFracs);
    HTR_NEED(rho); HTR_NEED(mu); HTR_NEED(lam); HTR_NEED(Di); HTR_NEED(SoS);
    hipStream_t s = (hipStream_t)stream;
@@ -567,7 +604,7 @@ int htr_update_properties(const htr_grid *g, const htr_fields *f, const htr_mix 
 
 int htr_update_conserved(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    HTR_NEED(MassFracs); HTR_NEED(temperature); HTR_NEED(velocity); HTR_NEED(rho); HTR_NEED(Conserved);
    hipStream_t s = (hipStream_t)stream;
    switch (kind_of(m)) {
@@ -580,7 +617,7 @@ int htr_update_conserved(const htr_grid *g, const htr_fields *f, const htr_mix *
 
 int htr_update_primitive(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    HTR_NEED(Conserved); HTR_NEED(temperature); HTR_NEED(pressure); HTR_NEED(MolarFracs); HTR_NEED(velocity);
    hipStream_t s = (hipStream_t)stream;
    switch (kind_of(m)) {
@@ -593,7 +630,7 @@ int htr_update_primitive(const htr_grid *g, const htr_fields *f, const htr_mix *
 
 int htr_ducros_sensor(const htr_grid *g, const htr_fields *f, double vorticityScale, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return done(rc);
    HTR_NEED(velocity); HTR_NEED(DucrosSensor);
    for (int d = 0; d < 3; d++) { HTR_NEED(nType[d]); HTR_NEED(m_d[d]); }
    return ducros_launch(fr, geo, vorticityScale, (hipStream_t)stream);
@@ -601,7 +638,7 @@ int htr_ducros_sensor(const htr_grid *g, const htr_fields *f, double vorticitySc
 
 int htr_shock_sensor(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    if (dir < 0 || dir > 2) return set_error(HTR_E_ARG, "bad direction %d", dir);
    HTR_NEED(Conserved); HTR_NEED(DucrosSensor); HTR_NEED(nType[dir]); HTR_NEED(shockSensor[dir]);
    HTR_DISPATCH3(shock_sensor_launch, dir, fr, geo, (hipStream_t)stream);
@@ -614,7 +651,7 @@ int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
    htr_view dec = {};
    if (diag && dir >= 0 && dir <= 2) dec = diag->decisions[dir];
    const int esz4 = 4;
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &dec, 1, &esz4); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &dec, 1, &esz4); if (rc) return done(rc);
    if (dir < 0 || dir > 2) return set_error(HTR_E_ARG, "bad direction %d", dir);
    if (scheme < HTR_EULER_HYBRID || scheme > HTR_EULER_KG) return set_error(HTR_E_ARG, "bad scheme %d", scheme);
    HTR_NEED(Conserved); HTR_NEED(rho); HTR_NEED(MassFracs); HTR_NEED(velocity); HTR_NEED(pressure);
@@ -626,7 +663,7 @@ int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
 
 int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    if (dir < 0 || dir > 2) return set_error(HTR_E_ARG, "bad direction %d", dir);
    HTR_NEED(Conserved); HTR_NEED(temperature); HTR_NEED(MolarFracs); HTR_NEED(rho); HTR_NEED(mu);
    HTR_NEED(lam); HTR_NEED(Di); HTR_NEED(velocity); HTR_NEED(Conserved_t);
@@ -637,7 +674,7 @@ int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const ht
 
 int htr_chem_source(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    HTR_NEED(rho); HTR_NEED(pressure); HTR_NEED(temperature); HTR_NEED(MassFracs); HTR_NEED(Conserved_t);
    hipStream_t s = (hipStream_t)stream;
    switch (kind_of(m)) {
@@ -654,7 +691,7 @@ int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int preci
    htr_view ns = {};
    if (diag) ns = diag->nsteps;
    const int esz4 = 4;
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &ns, 1, &esz4); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &ns, 1, &esz4); if (rc) return done(rc);
    if (precision != HTR_CHEM_FP64 && precision != HTR_CHEM_FP32_IR) return set_error(HTR_E_ARG, "bad precision %d", precision);
    HTR_NEED(Conserved); HTR_NEED(Conserved_t); HTR_NEED(Conserved_t_old); HTR_NEED(temperature);
    switch (kind_of(m)) {
@@ -672,8 +709,10 @@ int htr_chem_launch_config(int block, int blocks_per_cu)
 int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mix *m,
                             double *result, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
    if (!result) return set_error(HTR_E_ARG, "null result");
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo);
+   if (rc > 0) *result = 0.0;                  // empty rect: maximum over no cells
+   if (rc) return done(rc);
    HTR_NEED(velocity); HTR_NEED(SoS); HTR_NEED(rho); HTR_NEED(mu); HTR_NEED(lam); HTR_NEED(Di);
    HTR_NEED(temperature); HTR_NEED(MassFracs);
    for (int d = 0; d < 3; d++) HTR_NEED(m_d[d]);
@@ -688,14 +727,14 @@ int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mi
 
 int htr_rk_update(const htr_grid *g, const htr_fields *f, int nEq, int stage, double dt, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return done(rc);
    HTR_NEED(Conserved); HTR_NEED(Conserved_old); HTR_NEED(Conserved_t);
    return rk_update_launch(fr, geo, nEq, stage, dt, (hipStream_t)stream);
 }
 
 int htr_rk_init(const htr_grid *g, const htr_fields *f, int nEq, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return done(rc);
    HTR_NEED(Conserved); HTR_NEED(Conserved_old);
    return rk_init_launch(fr, geo, nEq, (hipStream_t)stream);
 }
@@ -703,7 +742,7 @@ int htr_rk_init(const htr_grid *g, const htr_fields *f, int nEq, void *stream)
 int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
                  double deltaTime, int chem_source, const htr_mix *m, void *stream)
 {
-   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return rc;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    HTR_NEED(Conserved_t);
    rc = zero_rhs_launch(fr, geo, m->host.nSpec + 4, (hipStream_t)stream);
    if (rc) return rc;

@@ -965,7 +965,7 @@ euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    // are staged when the chunk is deposited.
    // (TENO-A keeps no values across the compute: its reconstruction needs
    // the registers, so it stages each chunk straight into LDS)
-   constexpr bool PF = SCHEME != HTR_EULER_TENOA;
+   constexpr bool PF = false;
    double pv[PF ? MV : 1][6], rv[PF ? MR : 1];
    int tv = 0;
    auto fetch = [&](uint32_t c) {

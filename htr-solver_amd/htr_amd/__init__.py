@@ -56,7 +56,7 @@ EXPORTS = [
     "htr_ducros_sensor", "htr_shock_sensor", "htr_euler_flux", "htr_diffusion_flux",
     "htr_chem_source", "htr_chem_update", "htr_max_spectral_radius",
     "htr_rk_update", "htr_rk_init", "htr_full_rhs",
-    "htr_profile", "htr_profile_report", "htr_chem_launch_config",
+    "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
 ]
 
 _lib = None
@@ -82,6 +82,7 @@ def _declare(L):
         "htr_init": (i, [i]), "htr_last_error": (C.c_char_p, []), "htr_version": (C.c_char_p, []),
         "htr_profile": (i, [i]), "htr_profile_report": (i, [C.c_char_p, C.c_size_t]),
         "htr_chem_launch_config": (i, [i, i]),
+        "htr_stencil_table": (i, [i, i, i, dp]),
         "htr_mix_create_constprop": (i, [C.POINTER(vp), d, d, d, i, d, d, d]),
         "htr_mix_create_airmix": (i, [C.POINTER(vp), d, d, d, i, ip, dp]),
         "htr_mix_create_species": (i, [C.POINTER(vp), i, dp, fp, ip, dp, dp, dp, ip, dp]),

@@ -118,6 +118,15 @@ int htr_profile(int enable);
  * "name launches total_ms", then forgets them */
 int htr_profile_report(char *buf, size_t len);
 
+/* ---- stencil tables (no reference counterpart: read-back for tests) ------- */
+/* Entry k of node type `node` (src/prometeo_metric_coeffs.h:55-67 order) of the
+ * library's host copy of the tables it uploads to every device
+ * (src/prometeo_metric_coeffs.cc:42-209): table 0 Cp (cm2, cm1, cp1, cp2, cp3),
+ * 1 Recon_Plus, 2 Recon_Minus, 3 Coeffs_Plus, 4 Coeffs_Minus, 5 Interp,
+ * 6 Grad, 7 KennedyOrder (k = 0), 8 KennedyCoeff, 9 KennedyNSum (k = 0),
+ * 10 p10 (node ignored).  Needs no device. */
+int htr_stencil_table(int table, int node, int k, double *value);
+
 /* ---- mixtures (src/prometeo_mixture.cu:33-47 LoadMixtureTask) -------------- */
 /* ConstPropMix (src/Mixtures/ConstPropMix.inl:30-50); visc_model 0 constant
  * (p0 = viscosity), 1 power law (p0 = TempRef, p1 = ViscRef), 2 Sutherland

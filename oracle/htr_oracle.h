@@ -228,6 +228,20 @@ void   o_rosenbrock_testprob(double *x, double dtTry, double DelT, int32_t *nste
 int    o_chem_cell(const o_mix *m, double *x, const double *Conserved_t_old,
                    double *temperature, double dt);
 
+/* ---- test hooks: internal steps of the flux reconstruction -------------- */
+/* Roe state (ComputeRoeAverages, src/prometeo_rhs.inl:200-279) of the left and
+ * right states L, R = [rho, u, v, w, p, rhoE, Y_0 .. Y_ns-1];
+ * out = [a, e, H, a2, rho, v0, v1, v2, dpde, Y[ns], dpdrhoi[ns]] */
+void   o_test_roe(const o_mix *m, const double *L, const double *R, double *out);
+/* projectToCharacteristicSpace (inverse = 0) or projectFromCharacteristicSpace
+ * (inverse = 1) of q[nEq] along dir for a Roe state packed as above
+ * (src/prometeo_rhs.inl:436-582) */
+void   o_test_char(const o_mix *m, int dir, const double *roe, const double *q, double *r, int inverse);
+/* FluxReconstruction<Op> of the face on the + side of cell (x, y, z); op 0
+ * TENO, 1 TENO-A, 2 TENO-LAD; dec receives the face's decision word */
+void   o_test_face_flux(const o_grid *g, const o_fields *f, int dir, int op, int x, int y, int z,
+                        double RK0, double RK1, double dt, const o_mix *m, double *Flux, uint32_t *dec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -619,3 +619,89 @@ void o_diffusion_flux(const o_grid *g, const o_fields *f, int dir, const o_mix *
          for (int i = 0; i < cx; i++)
             diffusion_span(&c, i, j, k);
 }
+
+/* ------------------------------- test hooks -------------------------------
+ * Entry points into internal steps of the flux reconstruction, so that the
+ * CPU tests can pin them with closed forms (L R = I of the characteristic
+ * projections, the Roe state of equal left and right states, positivity of
+ * the limited face flux).  They run the same static functions as the tasks.
+ */
+
+/* two-cell stand-in box holding the left and right states of a face */
+typedef struct {
+   o_grid g;
+   o_fields f;
+   double rho[2], p[2], vel[6], cons[2*O_NEQ_MAX], Y[2*O_NSP_MAX];
+} pair_box;
+
+static void pair_init(pair_box *b, const o_mix *m, const double *L, const double *R)
+{
+   const int ns = m->nSpec;
+   memset(b, 0, sizeof(*b));
+   b->g.nx = 2; b->g.ny = 1; b->g.nz = 1;
+   const double *s[2] = { L, R };
+   for (int k = 0; k < 2; k++) {
+      b->rho[k] = s[k][0];
+      for (int i = 0; i < 3; i++) b->vel[2*i + k] = s[k][1 + i];
+      b->p[k] = s[k][4];
+      b->cons[2*(ns + 3) + k] = s[k][5];
+      for (int i = 0; i < ns; i++) b->Y[2*i + k] = s[k][6 + i];
+   }
+   b->f.rho = b->rho; b->f.pressure = b->p; b->f.velocity = b->vel;
+   b->f.Conserved = b->cons; b->f.MassFracs = b->Y;
+}
+
+static void roe_pack(const roe_t *R, int ns, double *out)
+{
+   out[0] = R->a; out[1] = R->e; out[2] = R->H; out[3] = R->a2; out[4] = R->rho;
+   for (int i = 0; i < 3; i++) out[5 + i] = R->vel[i];
+   out[8] = R->dpde;
+   for (int i = 0; i < ns; i++) { out[9 + i] = R->Yi[i]; out[9 + ns + i] = R->dpdrhoi[i]; }
+}
+
+static void roe_unpack(roe_t *R, int ns, const double *in)
+{
+   R->a = in[0]; R->e = in[1]; R->H = in[2]; R->a2 = in[3]; R->rho = in[4];
+   for (int i = 0; i < 3; i++) R->vel[i] = in[5 + i];
+   R->dpde = in[8];
+   for (int i = 0; i < ns; i++) { R->Yi[i] = in[9 + i]; R->dpdrhoi[i] = in[9 + ns + i]; }
+}
+
+void o_test_roe(const o_mix *m, const double *L, const double *R, double *out)
+{
+   pair_box b;
+   pair_init(&b, m, L, R);
+   ctx c;
+   setup(&c, &b.g, &b.f, m, 0);
+   roe_t r;
+   pt pl = { { 0, 0, 0 } }, pr = { { 1, 0, 0 } };
+   roe_averages(&c, &r, pl, pr);
+   roe_pack(&r, m->nSpec, out);
+}
+
+void o_test_char(const o_mix *m, int dir, const double *roe, const double *q, double *r, int inverse)
+{
+   o_grid g;
+   memset(&g, 0, sizeof(g));
+   g.nx = g.ny = g.nz = 1;
+   o_fields f;
+   memset(&f, 0, sizeof(f));
+   ctx c;
+   setup(&c, &g, &f, m, dir);
+   roe_t R;
+   roe_unpack(&R, m->nSpec, roe);
+   if (inverse) from_char(&c, r, q, &R);
+   else to_char(&c, r, q, &R);
+}
+
+void o_test_face_flux(const o_grid *g, const o_fields *f, int dir, int op, int x, int y, int z,
+                      double RK0, double RK1, double dt, const o_mix *m, double *Flux, uint32_t *dec)
+{
+   ctx c;
+   setup(&c, g, f, m, dir);
+   pt p = { { x, y, z } };
+   const int t = ntype(&c, dir, p);
+   const pt pP1 = wplus(g, dir, p, o_coeff_Cp(t, 2));
+   const double *me = f->m_e[dir];
+   teno_flux(&c, op, Flux, p, t, RK0, RK1, dt*(me[lin(&c, p)] + me[lin(&c, pP1)]), dec);
+}

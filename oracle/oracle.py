@@ -123,6 +123,10 @@ def _declare(L):
         "o_lu_solve": (None, [i, _dp, _dp]),
         "o_rosenbrock_testprob": (None, [_dp, d, d, C.POINTER(C.c_int32)]),
         "o_chem_cell": (i, [vp, _dp, _dp, _dp, d]),
+        "o_test_roe": (None, [vp, _dp, _dp, _dp]),
+        "o_test_char": (None, [vp, i, _dp, _dp, _dp, i]),
+        "o_test_face_flux": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, i, i, i, i, d, d, d, vp, _dp,
+                                    C.POINTER(C.c_uint32)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -416,3 +420,27 @@ def teno_reconstruct(op, side, y6, nType=0):
     bits = (C.c_int * 4)()
     v = lib().o_teno_reconstruct(op, side, dptr(_arr(y6)), nType, bits)
     return v, list(bits)
+
+
+# ---- test hooks: internal steps of the flux reconstruction -------------------
+def roe_state(m, L, R):
+    """Roe state of two cell states [rho, u, v, w, p, rhoE, Y...]; returns the
+    packed vector [a, e, H, a2, rho, v0, v1, v2, dpde, Y[ns], dpdrhoi[ns]]."""
+    out = np.zeros(9 + 2 * m.nSpec)
+    lib().o_test_roe(m.ptr, dptr(_arr(L)), dptr(_arr(R)), dptr(out))
+    return out
+
+
+def char_project(m, d, roe, q, inverse=False):
+    r = np.zeros(m.nSpec + 4)
+    lib().o_test_char(m.ptr, d, dptr(_arr(roe)), dptr(_arr(q)), dptr(r), int(inverse))
+    return r
+
+
+def face_flux(g, f, d, op, cell, RK0, RK1, dt, m):
+    """FluxReconstruction<Op> of the face on the + side of cell (x, y, z)."""
+    F = np.zeros(m.nSpec + 4)
+    dec = C.c_uint32(0)
+    lib().o_test_face_flux(C.byref(g.c()), C.byref(f.c()), d, op, cell[0], cell[1], cell[2], RK0, RK1, dt,
+                           m.ptr, dptr(F), C.byref(dec))
+    return F, dec.value

@@ -64,11 +64,31 @@ def euler_case(state, meta, n, om, d, scheme, RK=(1.0, 0.0), dt=1e-3, exec_lo=(0
     return f["Conserved_t"], ct, dec_o, dec_g
 
 
-def check_euler(ref, gpu, dec_o, dec_g, label, d, floor=None):
-    # a cell is affected by a flipped decision if either of its two faces flipped
-    flip_face = dec_o != dec_g
+# Decision flips allowed per kind (hybrid branch, limiter, TENO stencil
+# selection): at most 0.1 % of the faces of the sweep, and never more than
+# one face on the small boxes.  The library builds without FMA contraction,
+# like the oracle, so a flip can only come from the last-ulp differences of
+# the device's log/exp/pow in the Roe state's temperature inversion.
+FLIP_FRACTION = 1e-3
+
+
+def flip_counts(dec_o, dec_g, faces):
+    """(hybrid branch, limiter, TENO selection) flips among the given faces."""
+    x = (dec_o ^ dec_g)[faces]
+    both_teno = ((dec_o[faces] | dec_g[faces]) & 1) == 0
+    return int(np.sum(x & 1)), int(np.sum((x & 2) != 0)), int(np.sum(((x >> 2) != 0) & both_teno))
+
+
+def check_euler(ref, gpu, dec_o, dec_g, label, d, floor=None, faces=None, record=None):
+    """faces: mask of the faces (cells whose + face the sweep forms) that the
+    decision words cover; the flipped cells are reported and excluded from
+    the 1e-12 bound, and their number is bounded per kind."""
+    if faces is None:
+        faces = np.ones(dec_o.shape, bool)
+    flip_face = (dec_o != dec_g) & faces
     flip_cell = flip_face | np.roll(flip_face, 1, axis=2 - d)
     nflip = int(flip_face.sum())
+    kinds = flip_counts(dec_o, dec_g, faces)
     e = norm_err(gpu, ref, mask=~flip_cell, floor=floor)
     e_all = norm_err(gpu, ref, floor=floor)
     if nflip:
@@ -76,11 +96,18 @@ def check_euler(ref, gpu, dec_o, dec_g, label, d, floor=None):
         for i in idx:
             a, b = int(dec_o[tuple(i)]), int(dec_g[tuple(i)])
             say("   flip at %s: oracle %08x gpu %08x xor %08x" % (tuple(i), a, b, a ^ b))
-    teno_faces = int(np.sum((dec_o & 1) == 0))
-    say("%s dir %d: max norm err %.3e (all cells %.3e), TENO faces %d, decision flips %d"
-        % (label, d, max(e), max(e_all), teno_faces, nflip))
+    nfaces = int(faces.sum())
+    teno_faces = int(np.sum(((dec_o & 1) == 0) & faces))
+    lim_faces = int(np.sum(((dec_o & 2) != 0) & faces))
+    say("%s dir %d: max norm err %.3e (all cells %.3e), faces %d, TENO %d, limited %d, "
+        "flips hybrid/limiter/TENO %d/%d/%d"
+        % (label, d, max(e), max(e_all), nfaces, teno_faces, lim_faces, *kinds))
+    if record is not None:
+        record(label, d, nfaces, teno_faces, lim_faces, kinds, max(e), max(e_all))
     assert max(e) <= TOL, (label, d, e)
     assert max(e_all) <= 1e-6, (label, d, e_all)
+    allowed = max(1, int(FLIP_FRACTION * nfaces))
+    assert max(kinds) <= allowed, (label, d, kinds, allowed)
     return nflip
 
 
@@ -152,25 +179,25 @@ def test_sensors_air(air16):
 
 
 @pytest.mark.parametrize("scheme,label", SCHEMES)
-def test_euler_tgv(tgv16, scheme, label):
+def test_euler_tgv(tgv16, scheme, label, parity_record):
     st, meta, n, om = tgv16
     say("case tgv16 euler", label)
     for d in range(3):
         ref, gpu, do, dg = euler_case(st, meta, n, om, d, scheme)
-        check_euler(ref, gpu, do, dg, "tgv16 " + label, d, floor=flux_scale(st, d))
+        check_euler(ref, gpu, do, dg, "tgv16 " + label, d, floor=flux_scale(st, d), record=parity_record)
 
 
 @pytest.mark.parametrize("scheme,label", SCHEMES)
-def test_euler_air(air16, scheme, label):
+def test_euler_air(air16, scheme, label, parity_record):
     st, meta, n, om = air16
     say("case air16 euler", label)
     dt = dt_cfl(st, meta, n)
     for d in range(3):
         ref, gpu, do, dg = euler_case(st, meta, n, om, d, scheme, RK=(0.75, 0.25), dt=dt * 0.25)
-        check_euler(ref, gpu, do, dg, "air16 " + label, d)
+        check_euler(ref, gpu, do, dg, "air16 " + label, d, record=parity_record)
 
 
-def test_euler_air_forced_teno_limiter(air16):
+def test_euler_air_forced_teno_limiter(air16, parity_record):
     """All faces TENO (sensor false everywhere) with a large stage dt so that
     the positivity limiter fires on some faces."""
     st, meta, n, om = air16
@@ -183,7 +210,8 @@ def test_euler_air_forced_teno_limiter(air16):
         ref, gpu, do, dg = euler_case(st2, meta, n, om, d, O.SCHEME_HYBRID, RK=(1.0, 0.0), dt=dt)
         lim = int(np.sum((do & 2) != 0))
         say("  limiter faces dir %d: %d" % (d, lim))
-        check_euler(ref, gpu, do, dg, "air16 forced TENO", d)
+        assert lim > 0, "the limiter never fired"
+        check_euler(ref, gpu, do, dg, "air16 forced TENO", d, record=parity_record)
 
 
 def test_diffusion(tgv16, air16):
@@ -350,7 +378,7 @@ def test_tgv_rk3_steps(tgv16):
     assert max(e) <= 1e-11
 
 
-def test_tile_subrect_absolute_coordinates(air16):
+def test_tile_subrect_absolute_coordinates(air16, parity_record):
     """A tile-like call: execution rect [3, 12]^3 inside a 16^3 periodic box
     whose Fluid_bounds start at absolute (100, -7, 40).  Euler (hybrid, every
     direction) and diffusion against the oracle on the same rect; cells
@@ -367,7 +395,7 @@ def test_tile_subrect_absolute_coordinates(air16):
         inside[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
         assert np.all(gpu[:, ~inside] == 0.0)
         m = inside
-        check_euler(ref * m, gpu * m, np.where(m, do, 0), np.where(m, dg, 0), "sub-rect hybrid", d)
+        check_euler(ref * m, gpu * m, do, dg, "sub-rect hybrid", d, faces=m, record=parity_record)
     for d in range(3):
         g, f = oracle_fields(st, 5, n)
         g = g.with_rect(lo, hi)
@@ -428,7 +456,7 @@ def test_concurrent_streams():
         assert d == 0.0
 
 
-def test_wall_bounded_node_types():
+def test_wall_bounded_node_types(parity_record):
     """y bounded by staggered walls on a stretched grid: every staggered node
     type (L_S, Lp1_S, Lp2_S, Rm3_S, Rm2_S, Rm1_S, R_S) on the Euler and
     diffusion paths, execution rect = interior cells in y."""
@@ -438,10 +466,14 @@ def test_wall_bounded_node_types():
     st, om = complete_state(st, meta, n)
     dt = dt_cfl(st, meta, n) * 0.3
     hi = (n[0] - 1, n[1] - 2, n[2] - 1)
+    # faces of the y sweep: the + faces of the interior cells and the face
+    # of the wall ghost below the first one
+    faces = np.zeros(n[::-1], bool)
+    faces[:, 0:n[1] - 1, :] = True
     for scheme, label in ((O.SCHEME_HYBRID, "hybrid"), (O.SCHEME_TENO, "TENO"), (O.SCHEME_KG, "KG")):
         ref, gpu, do, dg = euler_case(st, meta, n, om, 1, scheme, RK=(1.0, 0.0), dt=dt,
                                       exec_lo=(0, 1, 0), exec_hi=hi)
-        check_euler(ref, gpu, do, dg, "wall " + label, 1)
+        check_euler(ref, gpu, do, dg, "wall " + label, 1, faces=faces, record=parity_record)
     for d in range(3):
         g, f = oracle_fields(st, 5, n)
         g = g.with_rect((0, 1, 0), hi)

@@ -254,3 +254,40 @@ def test_operator_tables_golden():
             k = ["cm2", "cm1", "cp1", "cp2", "cp3"].index(slot)
             assert L.o_coeff_Cp(t, k) == off, (name, slot); checked += 1
     assert checked > 600
+
+
+LIBHTR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "htr-solver_amd",
+                      "libhtr_amd.so")
+
+
+@pytest.mark.skipif(not os.path.exists(LIBHTR), reason="library not built")
+def test_library_stencil_tables_golden():
+    """The product library's own tables (the host copy it uploads to the
+    device) against the reference's operator tables, entry by entry."""
+    import ctypes
+    import htr_amd
+    L = htr_amd.lib()
+    ops = load("operators.json")
+    v = ctypes.c_double(0.0)
+
+    def get(table, node, k):
+        assert L.htr_stencil_table(table, node, k, ctypes.byref(v)) == 0
+        return v.value
+    checked = 0
+    for name, d in ops.items():
+        t = NODE.index(name)
+        for table, key in ((1, "ReconPlus"), (2, "ReconMinus"), (3, "CoeffsPlus"), (4, "CoeffsMinus"),
+                           (5, "Interp"), (6, "Grad"), (8, "KennedyCoeff")):
+            for k, val in d[key].items():
+                assert get(table, t, int(k)) == val, (name, key, k)
+                checked += 1
+        if "KennedyOrder" in d:
+            assert get(7, t, 0) == d["KennedyOrder"], name
+            checked += 1
+        for slot, off in d["Cp"].items():
+            k = ["cm2", "cm1", "cp1", "cp2", "cp3"].index(slot)
+            assert get(0, t, k) == off, (name, slot)
+            checked += 1
+    for k in range(14):
+        assert get(10, 0, k) == O.lib().o_coeff_p10(k)
+    assert checked > 600

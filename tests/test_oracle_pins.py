@@ -1,0 +1,296 @@
+"""Closed-form pins of the oracle's characteristic-space flux reconstruction,
+Roe state, sensors, viscous stress and positivity limiter (CPU only).
+
+Each check is an identity of the mathematics that the reference algorithm
+(src/prometeo_rhs.inl:200-759, src/prometeo_metric.inl:352-1038,
+src/prometeo_sensor.inl:50-77, src/prometeo_rhs.inl:1058-1215) must satisfy,
+chosen so that a plausible slip in the restatement -- a wrong eigenvector
+entry, a sign in the Lax-Friedrichs split, a dropped term of the sound speed,
+a swapped stencil row, a wrong blend -- fails it:
+
+  TENO all kept      where every candidate stencil is kept, the TENO face
+                     flux is -C6(f) of the physical flux f, whatever the Roe
+                     state: -(C6(L f + lam L q) + C6(L f - lam L q))/2 back-
+                     projected is -R L C6(f) = -C6(f), C6 being the 6th-order
+                     interpolation [1,-8,37,37,-8,1]/60.  Pins the projections,
+                     the split and the back-projection together.
+  L R = I            the back-projection inverts the projection for Roe states
+                     of arbitrary cell pairs.
+  Roe(L, L)          equal states give the cell state back, with a^2 = gamma
+                     p / rho of the ideal-gas mixture (the sound speed the
+                     properties task stores).
+  Ducros             a pure rotation gives exactly 0, a pure dilatation ~1.
+  TENO-A sensor      constant rho Y_i is smooth everywhere; a step is flagged
+                     exactly in the cells whose six-point stencil straddles it.
+  TENO step          every candidate stencil straddling a step at the face is
+                     dropped (only the upwind three-point stencil is kept, and
+                     the face value is the upwind constant), for TENO, TENO-A
+                     and TENO-LAD, both sides.
+  viscous stress     u = a x^2: d(sigma_xx)/dx = (8/3) mu a; u = a y^2:
+                     d(sigma_xy)/dy = 2 mu a (staggered 2nd-order differences
+                     are exact on quadratics).
+  limiter            a limited face flux is a convex blend theta HO + (1 -
+                     theta) LO of the unlimited flux and the first-order
+                     Lax-Friedrichs flux built independently from the
+                     projections, and the predicted partial densities on both
+                     sides are >= min(1e-60, first-order prediction).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from htr_amd import cases
+from hotpath import air_state, complete_state, dt_cfl, oracle_fields, oracle_mix, tgv_state
+
+C6 = np.array([1.0, -8.0, 37.0, 37.0, -8.0, 1.0]) / 60.0
+
+
+def mesh(n):
+    x = cases.cell_centres(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    return tuple(a.transpose(2, 1, 0) for a in (X, Y, Z))          # (z, y, x)
+
+
+def smooth_state(n, kind):
+    X, Y, Z = mesh(n)
+    vel = np.stack([0.3 + 0.2 * np.sin(X) * np.cos(Y), -0.1 + 0.2 * np.cos(Y + Z), 0.15 * np.sin(Z + X)])
+    st = {}
+    if kind == "constprop":
+        _, meta = cases.tgv(n)
+        st["velocity"] = vel
+        st["pressure"] = 1.0 + 0.1 * np.sin(X + 2 * Y) * np.cos(Z)
+        st["temperature"] = 1.0 + 0.1 * np.cos(X) * np.sin(Y + Z)
+        st["MolarFracs"] = np.ones((1,) + X.shape)
+    else:
+        _, meta = cases.air_box(n, 0)
+        Yb = np.array([0.6, 0.2, 0.1, 0.05, 0.05])[:, None, None, None]
+        ph = 0.7 * np.arange(5)[:, None, None, None]
+        Ym = Yb * (1 + 0.2 * np.sin(X[None] + ph) * np.cos(Y - Z)[None])
+        Ym /= Ym.sum(axis=0, keepdims=True)
+        W = cases.AIR_W[:, None, None, None]
+        MixW = 1.0 / np.sum(Ym / W, axis=0)
+        st["MolarFracs"] = Ym * MixW[None] / W
+        st["velocity"] = 5.0 * vel
+        st["pressure"] = 0.1 * (1.0 + 0.1 * np.sin(X + 2 * Y) * np.cos(Z))
+        st["temperature"] = 8.0 * (1.0 + 0.1 * np.cos(X) * np.sin(Y + Z))
+    cases.set_metrics(st, [cases.periodic_metrics(n)] * 3, (n, n, n))
+    return st, meta
+
+
+def all_kept_word(neq):
+    """Decision word of a TENO face whose 2 nEq reconstructions kept every stencil."""
+    h = 0
+    for _ in range(2 * neq):
+        h = (h * 17 + 15) & 0x3fffffff
+    return h << 2
+
+
+@pytest.mark.parametrize("kind", ["constprop", "airmix"])
+@pytest.mark.parametrize("scheme", [O.SCHEME_TENO, O.SCHEME_TENOA, O.SCHEME_TENOLAD])
+def test_teno_all_kept_is_minus_central6_of_physical_flux(kind, scheme):
+    n = 32
+    st, meta = smooth_state(n, kind)
+    st, om = complete_state(st, meta, (n, n, n), sensors=False)
+    ns = om.nSpec
+    for d in range(3):
+        g, f = oracle_fields(st, ns, (n, n, n))
+        f["Conserved_t"] = 0.0
+        dec = np.zeros(f["rho"].shape, np.uint32)
+        O.euler_flux(g, f, d, scheme, 1.0, 0.0, 1e-6, om, dec)
+        ax = 3 - d
+        Q = st["Conserved"]
+        u = st["velocity"][d]
+        p = st["pressure"]
+        F = Q * u[None]
+        F[ns + d] += p
+        F[ns + 3] += p * u
+        Fc = sum(C6[k] * np.roll(F, -(k - 2), axis=ax) for k in range(6))
+        ref = -(Fc - np.roll(Fc, 1, axis=ax)) * st["m_e%d" % d][None]
+        kept = dec == all_kept_word(ns + 4)
+        cells = kept & np.roll(kept, 1, axis=2 - d)
+        assert cells.mean() > 0.4, cells.mean()
+        err = np.max(np.abs(f["Conserved_t"] - ref) * cells[None], axis=(1, 2, 3)) / np.max(np.abs(ref), axis=(1, 2, 3))
+        assert np.all(err < 1e-12), (d, err)
+
+
+def cell_vector(st, ns, idx):
+    z, y, x = idx
+    return np.concatenate([[st["rho"][z, y, x]], st["velocity"][:, z, y, x],
+                           [st["pressure"][z, y, x], st["Conserved"][ns + 3, z, y, x]], st["MassFracs"][:, z, y, x]])
+
+
+@pytest.fixture(scope="module")
+def air8():
+    st, meta = air_state(8, seed=2)
+    st, om = complete_state(st, meta, (8, 8, 8), sensors=False)
+    return st, om
+
+
+def test_back_projection_inverts_projection(air8):
+    st, om = air8
+    ns = om.nSpec
+    rng = np.random.default_rng(1)
+    for _ in range(30):
+        a, b = tuple(rng.integers(0, 8, 3)), tuple(rng.integers(0, 8, 3))
+        roe = O.roe_state(om, cell_vector(st, ns, a), cell_vector(st, ns, b))
+        for d in range(3):
+            q = rng.normal(size=ns + 4)
+            r = O.char_project(om, d, roe, q)
+            back = O.char_project(om, d, roe, r, inverse=True)
+            assert np.max(np.abs(back - q)) <= 1e-12 * np.max(np.abs(q))
+            # and the other order: L R = I
+            again = O.char_project(om, d, roe, O.char_project(om, d, roe, q, inverse=True))
+            assert np.max(np.abs(again - q)) <= 1e-12 * np.max(np.abs(q))
+
+
+def test_roe_of_equal_states_is_the_cell_state(air8):
+    st, om = air8
+    ns = om.nSpec
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        idx = tuple(rng.integers(0, 8, 3))
+        z, y, x = idx
+        v = cell_vector(st, ns, idx)
+        roe = O.roe_state(om, v, v)
+        rho, vel, p, E = v[0], v[1:4], v[4], v[5]
+        assert roe[4] == rho and np.all(roe[5:8] == vel) and np.all(roe[9:9 + ns] == v[6:])
+        assert roe[1] == E / rho - 0.5 * float(np.dot(vel, vel))
+        assert roe[2] == (E + p) / rho
+        # ideal-gas mixture: a^2 = gamma p / rho = the stored sound speed squared
+        assert abs(roe[0] - st["SoS"][z, y, x]) <= 1e-13 * roe[0]
+
+
+def test_ducros_rotation_and_dilatation():
+    n = 16
+    X, Y, Z = mesh(n)
+    st, _ = tgv_state(n)
+    rot = dict(st, velocity=np.stack([np.sin(Y), np.sin(Z), np.sin(X)]))
+    g, f = oracle_fields(rot, 1, (n, n, n))
+    O.ducros_sensor(g, f, 1e-3)
+    assert np.all(f["DucrosSensor"] == 0.0)
+    dil = dict(st, velocity=100.0 * np.stack([np.sin(X), np.sin(Y), np.sin(Z)]))
+    g, f = oracle_fields(dil, 1, (n, n, n))
+    O.ducros_sensor(g, f, 1e-3)
+    D = f["DucrosSensor"]
+    assert np.all(D > 0.99999) and np.all(D <= 1.0)
+
+
+def test_tenoa_sensor_constant_and_step():
+    n = 16
+    st, meta = air_state(n, seed=0, composition="per-seed")
+    st, om = complete_state(st, meta, (n, n, n), sensors=False)
+    g, f = oracle_fields(st, 5, (n, n, n))
+    rhoY = np.broadcast_to(np.array([0.5, 0.3, 0.1, 0.05, 0.05])[:, None, None, None], (5, n, n, n)).copy()
+    f["Conserved"][:5] = rhoY
+    f["DucrosSensor"] = 1.0
+    for d in range(3):
+        O.shock_sensor(g, f, d, om)
+        assert np.all(f["shockSensor%d" % d] == 1), d
+    step = rhoY.copy()
+    step[:, :, :, n // 2:] *= 3.0
+    f["Conserved"][:5] = step
+    O.shock_sensor(g, f, 0, om)
+    rough = np.where(f["shockSensor0"][0, 0] == 0)[0]
+    # jumps between x = 7|8 and (periodically) 15|0: the sensor of cell p
+    # reads p - 2 .. p + 3, which straddles the jump for p = 5 .. 9 and
+    # p = 13 .. 15, 0, 1
+    assert list(rough) == [0, 1, 5, 6, 7, 8, 9, 13, 14, 15]
+    assert np.all(f["shockSensor0"] == f["shockSensor0"][:1, :1, :])
+
+
+@pytest.mark.parametrize("op", [0, 1, 2])
+@pytest.mark.parametrize("side", [0, 1])
+def test_teno_drops_stencils_across_a_step(op, side):
+    for jump in (1.0, -5.0, 1e3):
+        y = np.array([1.0, 1.0, 1.0, 1.0 + jump, 1.0 + jump, 1.0 + jump])
+        v, bits = O.teno_reconstruct(op, side, y)
+        # the three-point stencil on the upwind side of the face is the only
+        # one that does not straddle the step between points 2 and 3
+        assert bits == [0, 0, 1, 0], (op, side, jump, bits)
+        upwind = 1.0 if side == 0 else 1.0 + jump
+        assert abs(v - upwind) <= 1e-14 * max(1.0, abs(upwind))
+
+
+def viscous_case(n, dir_grad, comp, a=0.37):
+    st, meta = tgv_state(n)
+    X, Y, Z = mesh(n)
+    coord = (X, Y, Z)[dir_grad]
+    vel = np.zeros((3,) + X.shape)
+    vel[comp] = a * coord ** 2
+    st["velocity"] = vel
+    p0 = float(np.mean(st["pressure"]))
+    st["pressure"] = np.full_like(X, p0)
+    st["temperature"] = np.full_like(X, p0)
+    st, om = complete_state(st, meta, (n, n, n), sensors=False)
+    g, f = oracle_fields(st, 1, (n, n, n))
+    f["Conserved_t"] = 0.0
+    O.diffusion_flux(g, f, dir_grad, om)
+    return f["Conserved_t"], meta["visc"]
+
+
+def test_viscous_stress_closed_forms():
+    n, a = 16, 0.37
+    ct, mu = viscous_case(n, 0, 0, a)                 # u = a x^2, sweep x
+    interior = ct[1][:, :, 2:n - 2]
+    assert np.max(np.abs(interior - 8.0 / 3.0 * mu * a)) <= 1e-12 * (8.0 / 3.0 * mu * a) * 1e3
+    ct, mu = viscous_case(n, 1, 0, a)                 # u = a y^2, sweep y
+    interior = ct[1][:, 2:n - 2, :]
+    assert np.max(np.abs(interior - 2.0 * mu * a)) <= 1e-12 * (2.0 * mu * a) * 1e3
+    # no normal stress from a shear flow, no mass diffusion of a single species
+    assert np.max(np.abs(ct[2][:, 2:n - 2, :])) <= 1e-15 and np.max(np.abs(ct[0])) == 0.0
+
+
+def test_limited_flux_is_a_positive_convex_blend():
+    n = 16
+    st, meta = air_state(n, seed=0)
+    st, om = complete_state(st, meta, (n, n, n))
+    ns = om.nSpec
+    dt = dt_cfl(st, meta, (n, n, n)) * 40.0
+    g, f = oracle_fields(st, ns, (n, n, n))
+    rng = np.random.default_rng(5)
+    for d in range(3):
+        dec = np.zeros((n, n, n), np.uint32)
+        O.euler_flux(g, f, d, O.SCHEME_TENO, 1.0, 0.0, dt, om, dec)
+        lim = np.argwhere(dec & 2)
+        assert len(lim) > 0
+        for (z, y, x) in lim[rng.choice(len(lim), size=min(40, len(lim)), replace=False)]:
+            F, w = O.face_flux(g, f, d, 0, (int(x), int(y), int(z)), 1.0, 0.0, dt, om)
+            HO, w0 = O.face_flux(g, f, d, 0, (int(x), int(y), int(z)), 1.0, 0.0, 1e-300, om)
+            assert (w & 2) and not (w0 & 2)
+            p = (z, y, x)
+            pp = list(p)
+            pp[2 - d] = (pp[2 - d] + 1) % n
+            pp = tuple(pp)
+            roe = O.roe_state(om, cell_vector(st, ns, p), cell_vector(st, ns, pp))
+            pts = []
+            for k in (-2, -1, 0, 1, 2, 3):
+                q = list(p)
+                q[2 - d] = (q[2 - d] + k) % n
+                pts.append(tuple(q))
+            un = np.array([st["velocity"][d][q] for q in pts])
+            c = np.array([st["SoS"][q] for q in pts])
+            lam = np.full(ns + 4, np.max(np.abs(un)))
+            lam[0], lam[-1] = np.max(np.abs(un - c)), np.max(np.abs(un + c))
+
+            def phys(q):
+                Q = st["Conserved"][(slice(None),) + q]
+                u, pr = st["velocity"][d][q], st["pressure"][q]
+                fl = Q * u
+                fl[ns + d] += pr
+                fl[ns + 3] += pr * u
+                return Q, fl
+            Qa, Fa = phys(p)
+            Qb, Fb = phys(pp)
+            Wp = O.char_project(om, d, roe, Fa) + lam * O.char_project(om, d, roe, Qa)
+            Wm = O.char_project(om, d, roe, Fb) - lam * O.char_project(om, d, roe, Qb)
+            LO = O.char_project(om, d, roe, -0.5 * (Wp + Wm), inverse=True)
+            k = int(np.argmax(np.abs(HO - LO)))
+            theta = (F[k] - LO[k]) / (HO[k] - LO[k])
+            assert -1e-12 <= theta <= 1.0 + 1e-12
+            assert np.max(np.abs(F - (theta * HO + (1 - theta) * LO))) <= 1e-13 * np.max(np.abs(F))
+            lim_f = dt * (st["m_e%d" % d][p] + st["m_e%d" % d][pp])
+            rp = st["Conserved"][:ns][(slice(None),) + p]
+            rq = st["Conserved"][:ns][(slice(None),) + pp]
+            pred = np.concatenate([rp + lim_f * F[:ns], rq - lim_f * F[:ns]])
+            pred_lo = np.concatenate([rp + lim_f * LO[:ns], rq - lim_f * LO[:ns]])
+            assert np.all(pred >= np.minimum(1e-60, pred_lo) - 1e-15 * np.max(np.abs(rp)))
