@@ -909,253 +909,208 @@ euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, 
    }
 }
 
-// Pass 2b with the face data staged in LDS: a block of nEq waves takes a
-// chunk of 64 TENO faces; all its threads load, with coalesced reads, the
-// values the components need (per face: the Roe state, the eigenvalue
-// maxima and the per-point species sums of the record; per stencil point:
-// the conserved vector, u_n and p) into LDS once, then wave i computes
-// component i of the 64 faces from LDS.
+// Passes 2b + 2c in one kernel: a block of three waves takes a chunk of 64
+// listed TENO faces at a time.
+//   stage       waves 1 and 2 deposit in LDS, per stencil point of every face
+//               of the chunk, the values the components need (conserved
+//               vector, u_n, p): 6 (nEq + 2) doubles per face;
+//   components  wave w forms components w, w + 3, w + 6, ... of the 64
+//               faces from LDS (the component index is wave-uniform, so every
+//               branch on it is uniform), each wave first forming the
+//               per-point species sums of the left-eigenvector products
+//               (rhs.inl:443-451) once for its components; components and
+//               stencil-selection bits go to LDS;
+//   finish      wave 0 back-projects the 64 faces (add_column, the
+//               reference's summation order), tests the limiter, writes the
+//               face fluxes and decision words and lists the faces that need
+//               the limiter blend,
+// the finish of chunk c overlapping the staging of chunk c + 1.  The Roe
+// record of a face (euler_roe_kernel) is read where it is used, with
+// coalesced loads (list order = record order).  39 KB of LDS per block
+// (air): four blocks, twelve waves per CU.
+constexpr int TENO_WAVES = 3;
+
 template<class MX>
-struct CharStage {
+struct TenoStage {
    static constexpr int NS = MX::NS, NEQ = MX::NEQ;
    static constexpr int NPV = NS + 6;                // per point: rhoY_0..NS-1, rhou 3, rhoE, u_n, p
-   static constexpr int NRV = 2*NS + 11;             // per face: Y NS, v 3, a, e, rho, dpde, a2, Lam 3, dpdrhoi NS
-   static constexpr int MV = (NPV + NEQ - 1)/NEQ;    // point values per thread (v = wave + m*nEq)
-   static constexpr int MR = (NRV + NEQ - 1)/NEQ;    // record values per thread
-   double pt[6*NPV][64];
-   double rc[NRV][64];
-   double sm[24][64];                                // per point k: the four species sums 4k..4k+3
-   int    t[64];
+   double  pt[6*NPV][64];
+   double  fc[NEQ][64];
+   uint8_t bits[NEQ][64];
 };
 
-// record slot of staged value r: Y 0..NS-1 -> rec 0..NS-1; v, a -> NS..NS+3;
-// e, rho, dpde -> NS+5..NS+7 (H skipped); a2; Lam1, Lam, LamN; dpdrhoi
-template<class MX>
-__device__ __forceinline__ int char_rec_slot(int r)
-{
-   constexpr int NS = MX::NS;
-   if (r < NS + 4) return r;
-   if (r < NS + 7) return r + 1;
-   if (r < NS + 8) return 2*NS + 8;
-   if (r < NS + 11) return 2*NS + 9 + (r - NS - 8);
-   return NS + 8 + (r - NS - 11);
-}
-
 template<class MX, int DIR, int SCHEME, bool LISTED>
-__global__ void __launch_bounds__(64*MX::NEQ)
-euler_char_lds_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b,
-                      int64_t nfaces, const int32_t *__restrict__ list, const int *__restrict__ counter)
+__global__ void __launch_bounds__(64*TENO_WAVES)
+euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b,
+                  double RK0, double RK1, double dt, double *__restrict__ F, int64_t nfaces,
+                  const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec,
+                  int32_t *__restrict__ llist, int *__restrict__ lcount)
 {
    using E_t = Euler<MX, DIR>;
-   using St = CharStage<MX>;
-   constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = St::NPV, NRV = St::NRV, MV = St::MV, MR = St::MR;
+   using St = TenoStage<MX>;
+   constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = St::NPV;
    constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
    constexpr int iN = E_t::iN;
    __shared__ St S;
    const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
    const uint32_t nchunks = (n + 63u)/64u;
    const int lane = threadIdx.x & 63;
-   const int wv = threadIdx.x >> 6;                   // wave = component
+   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+   const int64_t cap = b.cap;
    E_t E{f, *mix, g, hv};
-   // Stencil-point values of a chunk, fetched into registers one chunk ahead
-   // so that their scattered loads are in flight while the previous chunk
-   // computes: every wave forms its lane's face geometry and loads a share
-   // of the point values (the geometry loads first: the loads' counter
-   // completes in order).  The record values, read with coalesced loads,
-   // are staged when the chunk is deposited.
-   // (TENO-A keeps no values across the compute: its reconstruction needs
-   // the registers, so it stages each chunk straight into LDS)
-   constexpr bool PF = false;
-   double pv[PF ? MV : 1][6], rv[PF ? MR : 1];
-   int tv = 0;
-   auto fetch = [&](uint32_t c) {
+
+   auto face_of = [&](uint32_t w, P3 &q, int &t) {
+      q = Faces<DIR>::cell(g, f, LISTED ? (uint32_t)list[w] : w);
+      t = ld<int>(f.nType[DIR], q);
+   };
+
+   // waves 1 .. TENO_WAVES-1: point values of chunk c into LDS
+   auto stage = [&](uint32_t c) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
       P3 q = mkp(g, 0, 0, 0);
       int t = 0;
-      if (valid) {
-         q = Faces<DIR>::cell(g, f, LISTED ? (uint32_t)list[w] : w);
-         t = ld<int>(f.nType[DIR], q);
-      }
-      tv = t;                                     // Std_node (0) on lanes past the end
-      if constexpr (PF) {
-#pragma unroll
-         for (int m = 0; m < MR; m++) {
-            const int r = wv + m*NEQ;
-            rv[m] = (valid && r < NRV) ? b.rec[(int64_t)char_rec_slot<MX>(r)*b.cap + w] : 0.0;
-         }
-      }
-#pragma unroll
-      for (int m = 0; m < MV; m++) {
-         const int v = wv + m*NEQ;
+      if (valid) face_of(w, q, t);
+      P3 P[6];
+      if (valid) E.points(P, q, t);
+      for (int v = wv - 1; v < NPV; v += TENO_WAVES - 1) {
 #pragma unroll
          for (int k = 0; k < 6; k++) {
             double x = 0.0;
-            if (valid && v < NPV) {
-               const P3 P = (k == 2) ? q : (k < 2) ? wminus(g, DIR, q, c_st.Cp[t][k]) : wplus(g, DIR, q, c_st.Cp[t][k-1]);
-               x = (v < NS + 4) ? ld<double>(f.Conserved, P, v) : (v == NS + 4) ? ld<double>(f.velocity, P, iN)
-                                                                                  : ld<double>(f.pressure, P);
-            }
-            if constexpr (PF) pv[m][k] = x;
-            else if (v < NPV) S.pt[k*NPV + v][lane] = x;
-         }
-      }
-      if constexpr (!PF) {
-#pragma unroll
-         for (int m = 0; m < MR; m++) {
-            const int r = wv + m*NEQ;
-            if (r < NRV) S.rc[r][lane] = valid ? b.rec[(int64_t)char_rec_slot<MX>(r)*b.cap + w] : 0.0;
+            if (valid)
+               x = (v < NS + 4) ? ld<double>(f.Conserved, P[k], v) : (v == NS + 4) ? ld<double>(f.velocity, P[k], iN)
+                                                                      : ld<double>(f.pressure, P[k]);
+            S.pt[k*NPV + v][lane] = x;
          }
       }
    };
-   if (PF && blockIdx.x < nchunks) fetch(blockIdx.x);
-   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-      const uint32_t w = c*64u + (uint32_t)lane;
-      const bool valid = w < n;
-      // ---- deposit the fetched chunk and its record, then start fetching
-      // the next chunk
-      if constexpr (!PF) fetch(c);
-      else {
-#pragma unroll
-         for (int m = 0; m < MR; m++) {
-            const int r = wv + m*NEQ;
-            if (r < NRV) S.rc[r][lane] = rv[m];
-         }
-      }
-      if (wv == 0) S.t[lane] = tv;
-      if constexpr (PF) {
-#pragma unroll
-         for (int m = 0; m < MV; m++) {
-            const int v = wv + m*NEQ;
-            if (v < NPV) {
-#pragma unroll
-               for (int k = 0; k < 6; k++) S.pt[k*NPV + v][lane] = pv[m][k];
-            }
-         }
-      }
-      __syncthreads();
-      if (PF && c + gridDim.x < nchunks) fetch(c + gridDim.x);
-      // ---- the species sums of point k of the 64 faces (wave k, then k +
-      // nEq), the per-point part of projectToCharacteristicSpace
-      // (rhs.inl:443-451)
-      auto roe_of = [&](typename E_t::Roe &R) {
-#pragma unroll
-         for (int s = 0; s < NS; s++) { R.Y[s] = S.rc[s][lane]; R.dpdrhoi[s] = S.rc[NS + 11 + s][lane]; }
-#pragma unroll
-         for (int j = 0; j < 3; j++) R.v[j] = S.rc[NS + j][lane];
-         R.a = S.rc[NS + 3][lane];
-         R.H = 0.0;
-         R.e = S.rc[NS + 4][lane];
-         R.rho = S.rc[NS + 5][lane];
-         R.dpde = S.rc[NS + 6][lane];
-         R.a2 = S.rc[NS + 7][lane];
-      };
-      for (int k = __builtin_amdgcn_readfirstlane(wv); valid && k < 6; k += NEQ) {
-         typename E_t::Roe R;
-         roe_of(R);
-         typename E_t::Proj J;
-         E_t::proj(J, R);
-         const double un = S.pt[k*NPV + NS + 4][lane];
-         double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0;
-#pragma unroll
-         for (int j = 0; j < NS; j++) {
-            const double q = S.pt[k*NPV + j][lane];
-            const double wj = (J.Coeff - R.dpdrhoi[j])*J.iaRoe2;
-            s1q += wj*q;
-            sqq += q;
-            const double fq = q*un;
-            s1f += wj*fq;
-            sqf += fq;
-         }
-         S.sm[4*k + 0][lane] = s1q; S.sm[4*k + 1][lane] = sqq;
-         S.sm[4*k + 2][lane] = s1f; S.sm[4*k + 3][lane] = sqf;
-      }
-      __syncthreads();
-      // ---- compute: wave wv = component i of the 64 faces (i in a scalar
-      // register, so every branch on it is uniform)
-      const int tt = S.t[lane];
-      const bool all_std = __ballot(tt != Std_node) == 0ull;
-      if (valid && wv < NEQ) {
-         const int i = __builtin_amdgcn_readfirstlane(wv);
-         typename E_t::Roe R;
-         roe_of(R);
-         const double lam = S.rc[NS + 8 + ((i == 0) ? 0 : (i == NEQ-1) ? 2 : 1)][lane];
-         typename E_t::Proj J;
-         E_t::proj(J, R);
-         const double Ysel = (i >= 1 && i <= NS) ? S.rc[i-1][lane] : 0.0;
-         double vp[6], vm[6];
-#pragma unroll
-         for (int k = 0; k < 6; k++) {
-            const double *pv = &S.pt[k*NPV][0];
-            double sums[4];
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) sums[cc] = S.sm[4*k + cc][lane];
-            const double un = S.pt[k*NPV + NS + 4][lane];
-            const double pr = S.pt[k*NPV + NS + 5][lane];
-            const double qs = (i >= 1 && i <= NS) ? S.pt[k*NPV + i - 1][lane] : 0.0;
-            double qm[4], fm[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) { qm[j] = S.pt[k*NPV + NS + j][lane]; fm[j] = qm[j]*un; }
-            (void)pv;
-            fm[iN] += pr;
-            fm[3] += pr*un;
-            const double Q = E_t::char_row(i, R, J, sums[0], sums[1], qs, qm, Ysel);
-            const double F = E_t::char_row(i, R, J, sums[2], sums[3], qs*un, fm, Ysel);
-            vp[k] = F + lam*Q;
-            vm[k] = F - lam*Q;
-         }
-         uint32_t bp, bm;
-         double FPlus, FMinus;
-         if (OP != 1 && all_std) {
-            FPlus  = teno_reconstruct<OP, 0, true>(vp, tt, &bp);
-            FMinus = teno_reconstruct<OP, 1, true>(vm, tt, &bm);
-         } else {
-            FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
-            FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
-         }
-         b.Fc[(int64_t)i*b.cap + w] = -0.5*(FPlus + FMinus);
-         b.bits[(int64_t)i*b.cap + w] = (uint8_t)(bp | (bm << 4));
-      }
-      __syncthreads();
-   }
-}
 
-// Faces whose positivity test fails are appended to the limiter list llist
-// (count in lcount); the blend runs in euler_limit_kernel on those alone, so
-// that this pass does not hold the first-order flux's registers.
-template<class MX, int DIR, bool LISTED>
-__global__ void __launch_bounds__(256)
-euler_finish_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv,
-                    double RK0, double RK1,
-                    double dt, TenoBufs b, double *__restrict__ F, int64_t nfaces,
-                    const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec,
-                    int32_t *__restrict__ llist, int *__restrict__ lcount)
-{
-   const int64_t n = LISTED ? (int64_t)counter[1] : nfaces;
-   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
-   Euler<MX, DIR> E{f, *mix, g, hv};
-   const int64_t nround = (n + stride - 1)/stride*stride;      // whole waves run every round (ballot)
-   for (int64_t w = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; w < nround; w += stride) {
+   // wave 0: back-projection, limiter test, face flux and decision word of
+   // the 64 faces of chunk c (rhs.inl:708-721)
+   auto finish = [&](uint32_t c) {
+      const uint32_t w = c*64u + (uint32_t)lane;
       bool limit = false;
       if (w < n) {
-         const uint32_t fid = LISTED ? (uint32_t)list[w] : (uint32_t)w;
-         const P3 q = Faces<DIR>::cell(g, f, fid);
-         double Flux[MX::NEQ];
-         const uint32_t word = E.face_finish(Flux, b.rec, b.cap, w, b.Fc, b.bits, q, ld<int>(f.nType[DIR], q),
-                                             RK0, RK1, dt, &limit);
-         store_face<MX::NEQ>(F, nfaces, fid, Flux);
-         if (dec.ptr) at<uint32_t>(dec, q) = word;
+         P3 q;
+         int t;
+         face_of(w, q, t);
+         typename E_t::Roe R;
+         E_t::get_rec(b.rec, cap, w, R);
+         const double dedp = 1.0/R.dpde;
+         const double ERoe = R.e + 0.5*E_t::mod2(R.v);
+         double Flux[NEQ];
+         double sumQsp = 0.0;
+         uint32_t h = 0;
+#pragma unroll
+         for (int i = 0; i < NEQ; i++) {
+            E_t::add_column(Flux, sumQsp, i, S.fc[i][lane], R, ERoe, dedp);
+            const uint32_t bb = S.bits[i][lane];
+            h = fold_bits(fold_bits(h, bb & 15u), bb >> 4);
+         }
+         if (NS > 1) {
+            const P3 pP1 = wplus(g, DIR, q, c_st.Cp[t][2]);
+            const double lim_f = dt*(ld<double>(f.m_e[DIR], q) + ld<double>(f.m_e[DIR], pP1));
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+               const double dRhoi = lim_f*Flux[i];
+               if (((RK0*ld<double>(f.Conserved_old, q, i) + RK1*ld<double>(f.Conserved, q, i) + dRhoi) < -1e-8) ||
+                   ((RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*ld<double>(f.Conserved, pP1, i) - dRhoi) < -1e-8))
+                  limit = true;
+            }
+         }
+         const uint32_t fid = LISTED ? (uint32_t)list[w] : w;
+         store_face<NEQ>(F, nfaces, fid, Flux);
+         if (dec.ptr) at<uint32_t>(dec, q) = h << 2;
       }
       const uint64_t mask = __ballot(limit);
       if (mask) {
-         const int lane = threadIdx.x & 63;
          const int leader = __ffsll((unsigned long long)mask) - 1;
          int base = 0;
          if (lane == leader) base = atomicAdd(lcount, __popcll(mask));
          base = __shfl(base, leader, 64);
          if (limit) llist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)w;
       }
+   };
+
+   uint32_t last = 0;
+   bool any = false;
+   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+      if (wv > 0) stage(c);
+      else if (any) finish(last);
+      __syncthreads();
+      // ---- components wv, wv + TENO_WAVES, ... of the 64 faces
+      const uint32_t w = c*64u + (uint32_t)lane;
+      const bool valid = w < n;
+      int tt = 0;
+      if (valid) { P3 q; face_of(w, q, tt); }
+      const bool all_std = __ballot(tt != Std_node) == 0ull;
+      if (valid) {
+         typename E_t::Roe R;
+#pragma unroll
+         for (int j = 0; j < 3; j++) R.v[j] = b.rec[(int64_t)(NS + j)*cap + w];
+         R.a = b.rec[(int64_t)(NS + 3)*cap + w];
+         R.e = b.rec[(int64_t)(NS + 5)*cap + w];
+         R.rho = b.rec[(int64_t)(NS + 6)*cap + w];
+         R.dpde = b.rec[(int64_t)(NS + 7)*cap + w];
+         R.a2 = b.rec[(int64_t)(2*NS + 8)*cap + w];
+         typename E_t::Proj J;
+         E_t::proj(J, R);
+         // weights of the species sums of the left-eigenvector products
+         // (rhs.inl:443-451)
+         double wj[NS];
+#pragma unroll
+         for (int j = 0; j < NS; j++) wj[j] = (J.Coeff - b.rec[(int64_t)(NS + 8 + j)*cap + w])*J.iaRoe2;
+         for (int i = wv; i < NEQ; i += TENO_WAVES) {
+            // the point values are re-read from LDS and the species sums
+            // re-formed for every component: held in registers across the
+            // loop they would cost some 120 VGPRs
+            asm volatile("" ::: "memory");
+            const double lam = b.rec[(int64_t)(2*NS + ((i == 0) ? 9 : (i == NEQ-1) ? 11 : 10))*cap + w];
+            const double Ysel = (i >= 1 && i <= NS) ? b.rec[(int64_t)(i - 1)*cap + w] : 0.0;
+            double vp[6], vm[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+               const double un = S.pt[k*NPV + NS + 4][lane];
+               const double pr = S.pt[k*NPV + NS + 5][lane];
+               double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0, qs = 0.0;
+#pragma unroll
+               for (int j = 0; j < NS; j++) {
+                  const double qv = S.pt[k*NPV + j][lane];
+                  if (j == i - 1) qs = qv;
+                  s1q += wj[j]*qv;
+                  sqq += qv;
+                  const double fq = qv*un;
+                  s1f += wj[j]*fq;
+                  sqf += fq;
+               }
+               double qm[4], fm[4];
+#pragma unroll
+               for (int j = 0; j < 4; j++) { qm[j] = S.pt[k*NPV + NS + j][lane]; fm[j] = qm[j]*un; }
+               fm[iN] += pr;
+               fm[3] += pr*un;
+               const double Q = E_t::char_row(i, R, J, s1q, sqq, qs, qm, Ysel);
+               const double Fv = E_t::char_row(i, R, J, s1f, sqf, qs*un, fm, Ysel);
+               vp[k] = Fv + lam*Q;
+               vm[k] = Fv - lam*Q;
+            }
+            uint32_t bp, bm;
+            double FPlus, FMinus;
+            if (OP != 1 && all_std) {
+               FPlus  = teno_reconstruct<OP, 0, true>(vp, tt, &bp);
+               FMinus = teno_reconstruct<OP, 1, true>(vm, tt, &bm);
+            } else {
+               FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
+               FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
+            }
+            S.fc[i][lane] = -0.5*(FPlus + FMinus);
+            S.bits[i][lane] = (uint8_t)(bp | (bm << 4));
+         }
+      }
+      __syncthreads();
+      last = c;
+      any = true;
    }
+   if (wv == 0 && any) finish(last);
 }
 
 template<class MX, int DIR, bool LISTED>
@@ -1209,18 +1164,15 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
    constexpr int kMaxBlocks = 256*16;
    const TenoBufs b = teno_bufs<MX>(w.teno, nfaces);
    const unsigned nb = (unsigned)std::min<int64_t>(grid_for(nfaces, 256), kMaxBlocks);
-   const unsigned nbc = (unsigned)std::min<int64_t>(grid_for((nfaces + 63)/64*64*MX::NEQ, 256), kMaxBlocks);
    HTR_LAUNCH(HTR_DIR_NAME("euler_roe", DIR), (euler_roe_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
                       f, mix, g, hv, b, nfaces, w.list, w.counter);
    int rc = check_launch("euler_roe_kernel"); if (rc) return rc;
-   const unsigned nchunk = (unsigned)std::min<int64_t>((nfaces + 63)/64, 256*8);
-   HTR_LAUNCH(HTR_DIR_NAME("euler_char", DIR), (euler_char_lds_kernel<MX, DIR, SCHEME, LISTED>), dim3(nchunk),
-              dim3(64*MX::NEQ), 0, s, f, mix, g, hv, b, nfaces, w.list, w.counter);
-   rc = check_launch("euler_char_kernel"); if (rc) return rc;
    int32_t *llist = w.list + nfaces;                  // limiter list after the TENO list
-   HTR_LAUNCH(HTR_DIR_NAME("euler_finish", DIR), (euler_finish_kernel<MX, DIR, LISTED>), dim3(nb), dim3(256), 0, s,
-                      f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, w.counter, dec, llist, w.counter + 2);
-   if ((rc = check_launch("euler_finish_kernel"))) return rc;
+   const unsigned nchunk = (unsigned)std::min<int64_t>((nfaces + 63)/64, 256*8);
+   HTR_LAUNCH(HTR_DIR_NAME("euler_teno", DIR), (euler_teno_kernel<MX, DIR, SCHEME, LISTED>), dim3(nchunk),
+              dim3(64*TENO_WAVES), 0, s, f, mix, g, hv, b, RK0, RK1, dt, w.faces, nfaces, w.list, w.counter, dec,
+              llist, w.counter + 2);
+   if ((rc = check_launch("euler_teno_kernel"))) return rc;
    if (MX::NS == 1) return HTR_OK;                   // single species: no positivity limiter
    HTR_LAUNCH(HTR_DIR_NAME("euler_limit", DIR), (euler_limit_kernel<MX, DIR, LISTED>), dim3(256), dim3(256), 0, s,
               f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, llist, w.counter + 2, dec);
