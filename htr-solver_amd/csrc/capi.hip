@@ -42,7 +42,9 @@ static std::map<std::pair<int, uintptr_t>, Workspace*> g_ws;   // (device, strea
 static bool g_ready[16];
 
 // grows a workspace buffer; the old buffer is freed only after the stream's
-// earlier work (the only work that can use it) has finished
+// earlier work (the only work that can use it) has finished.  Called with
+// the workspace's own lock held, not the library's: a growth on one stream
+// does not stall calls on others.
 static int grow(hipStream_t s, void **p, int64_t &cap, int64_t need, size_t elem, const char *what)
 {
    if (need <= cap) return HTR_OK;
@@ -60,7 +62,7 @@ static int grow(hipStream_t s, void **p, int64_t &cap, int64_t need, size_t elem
 
 int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno)
 {
-   std::lock_guard<std::mutex> lk(g_mu);
+   std::lock_guard<std::mutex> lk(w.mu);
    int rc = grow(w.stream, (void**)&w.faces, w.faces_cap, ndoubles, sizeof(double), "face buffer");
    if (!rc) rc = grow(w.stream, (void**)&w.list, w.list_cap, nlist, sizeof(int32_t), "face list");
    if (!rc) rc = grow(w.stream, (void**)&w.teno, w.teno_cap, nteno, sizeof(double), "TENO face buffer");
@@ -69,7 +71,7 @@ int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno
 
 int workspace_enthalpy(Workspace &w, int64_t ndoubles)
 {
-   std::lock_guard<std::mutex> lk(g_mu);
+   std::lock_guard<std::mutex> lk(w.mu);
    return grow(w.stream, (void**)&w.hbuf, w.hbuf_cap, ndoubles, sizeof(double), "enthalpy buffer");
 }
 
@@ -81,12 +83,13 @@ Workspace *workspace(hipStream_t s)
    Workspace *&w = g_ws[std::make_pair(dev, (uintptr_t)s)];
    if (w) return w;
    Workspace *n = new Workspace();
-   memset(n, 0, sizeof(Workspace));
    n->stream = s;
    if (hipMalloc(&n->partial, sizeof(double)*Workspace::kPartial) != hipSuccess ||
        hipHostMalloc(&n->result_host, sizeof(double)) != hipSuccess ||
        hipMalloc(&n->counter, sizeof(int)*64) != hipSuccess) {
       set_error(HTR_E_DEVICE, "workspace allocation failed");
+      if (n->partial) (void)hipFree(n->partial);
+      if (n->result_host) (void)hipHostFree(n->result_host);
       delete n;
       return nullptr;
    }
@@ -658,7 +661,7 @@ int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
    HTR_NEED(nType[dir]); HTR_NEED(m_e[dir]); HTR_NEED(Conserved_t);
    if (scheme != HTR_EULER_KG) { HTR_NEED(SoS); if (m->host.nSpec > 1) HTR_NEED(Conserved_old); }
    if (scheme == HTR_EULER_HYBRID) HTR_NEED(shockSensor[dir]);
-   HTR_DISPATCH3(euler_dispatch, dir, scheme, fr, m->dev, geo, RK0, RK1, deltaTime, dec, (hipStream_t)stream);
+   HTR_DISPATCH3(euler_dispatch, dir, scheme, fr, m->dev, geo, RK0, RK1, host_dt(deltaTime), dec, (hipStream_t)stream);
 }
 
 int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
@@ -739,19 +742,46 @@ int htr_rk_init(const htr_grid *g, const htr_fields *f, int nEq, void *stream)
    return rk_init_launch(fr, geo, nEq, (hipStream_t)stream);
 }
 
-int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
-                 double deltaTime, int chem_source, const htr_mix *m, void *stream)
+int htr_workspace_reserve(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
 {
    Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
-   HTR_NEED(Conserved_t);
-   rc = zero_rhs_launch(fr, geo, m->host.nSpec + 4, (hipStream_t)stream);
+   WorkspaceNeed need;
+   switch (kind_of(m)) {
+      case K_CP:  need = euler_workspace_need<MixCP>(geo); break;
+      case K_AIR: need = euler_workspace_need<MixAir>(geo); break;
+      default: need = euler_workspace_need<Mix30>(geo); break;
+   }
+   Workspace *w = workspace((hipStream_t)stream);
+   if (!w) return HTR_E_DEVICE;
+   if ((rc = workspace_faces(*w, need.faces, need.list, need.teno))) return rc;
+   return workspace_enthalpy(*w, need.hbuf);
+}
+
+static int full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
+                    const DtArg &dt, int chem_source, const htr_mix *m, void *stream)
+{
+   htr_view dec = {};
+   const int esz4 = 4;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &dec, 1, &esz4); if (rc) return done(rc);
+   if (scheme < HTR_EULER_HYBRID || scheme > HTR_EULER_KG) return set_error(HTR_E_ARG, "bad scheme %d", scheme);
+   HTR_NEED(Conserved_t); HTR_NEED(Conserved); HTR_NEED(rho); HTR_NEED(MassFracs); HTR_NEED(velocity);
+   HTR_NEED(pressure);
+   for (int d = 0; d < 3; d++) { HTR_NEED(nType[d]); HTR_NEED(m_e[d]); }
+   if (scheme != HTR_EULER_KG) { HTR_NEED(SoS); if (m->host.nSpec > 1) HTR_NEED(Conserved_old); }
+   if (scheme == HTR_EULER_HYBRID) for (int d = 0; d < 3; d++) HTR_NEED(shockSensor[d]);
+   hipStream_t s = (hipStream_t)stream;
+   rc = zero_rhs_launch(fr, geo, m->host.nSpec + 4, s);
    if (rc) return rc;
    if (chem_source && !m->host.constprop) {
       rc = htr_chem_source(g, f, m, stream);
       if (rc) return rc;
    }
    for (int d = 2; d >= 0; d--) {
-      rc = htr_euler_flux(d, scheme, g, f, RK0, RK1, deltaTime, m, nullptr, stream);
+      switch (kind_of(m)) {
+         case K_CP:  rc = euler_dispatch<MixCP>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s); break;
+         case K_AIR: rc = euler_dispatch<MixAir>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s); break;
+         default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
+      }
       if (rc) return rc;
    }
    for (int d = 2; d >= 0; d--) {
@@ -759,6 +789,43 @@ int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0,
       if (rc) return rc;
    }
    return HTR_OK;
+}
+
+int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
+                 double deltaTime, int chem_source, const htr_mix *m, void *stream)
+{
+   return full_rhs(g, f, scheme, RK0, RK1, host_dt(deltaTime), chem_source, m, stream);
+}
+
+int htr_full_rhs_cfl(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
+                     double cfl, double scale, const double *d_radius, int chem_source, const htr_mix *m,
+                     void *stream)
+{
+   if (!d_radius) return set_error(HTR_E_ARG, "null radius");
+   DtArg dt;
+   dt.dt = 0.0; dt.radius = d_radius; dt.cfl = cfl; dt.scale = scale;
+   return full_rhs(g, f, scheme, RK0, RK1, dt, chem_source, m, stream);
+}
+
+int htr_rk_stage(const htr_grid *g, const htr_fields *f, int stage, double deltaTime, double cfl,
+                 const double *d_radius, const htr_mix *m, const htr_diag *diag, void *stream)
+{
+   htr_view fl = {};
+   if (diag) fl = diag->flags;
+   const int esz4 = 4;
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &fl, 1, &esz4); if (rc) return done(rc);
+   HTR_NEED(Conserved); HTR_NEED(Conserved_old); HTR_NEED(Conserved_t); HTR_NEED(temperature);
+   HTR_NEED(pressure); HTR_NEED(MolarFracs); HTR_NEED(velocity); HTR_NEED(MassFracs);
+   HTR_NEED(rho); HTR_NEED(mu); HTR_NEED(lam); HTR_NEED(Di); HTR_NEED(SoS);
+   DtArg dt = host_dt(deltaTime);
+   if (d_radius) { dt.radius = d_radius; dt.cfl = cfl; dt.scale = 1.0; }
+   hipStream_t s = (hipStream_t)stream;
+   switch (kind_of(m)) {
+      case K_CP:  return rk_stage_launch<MixCP>(fr, m->dev, geo, stage, dt, fl, s);
+      case K_AIR: return rk_stage_launch<MixAir>(fr, m->dev, geo, stage, dt, fl, s);
+      case K_30:  return rk_stage_launch<Mix30>(fr, m->dev, geo, stage, dt, fl, s);
+      default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
+   }
 }
 
 } // extern "C"

@@ -29,6 +29,20 @@ struct Geo {
    uint32_t cb[3];    // common stride triple (cx, cy, cz)
 };
 
+// Time step of a launch: the host value dt, or, when radius is set, the
+// ConstantCFL step (cfl / max spectral radius) read on the device from the
+// result of a device-side reduction and multiplied by scale, in the
+// reference's rounding order: dt = cfl/radius, then dt*scale
+// (src/prometeo.rg:1165, :1419).
+struct DtArg {
+   double dt;
+   const double *radius;
+   double cfl, scale;
+   __device__ double get() const { return radius ? (cfl/(*radius))*scale : dt; }
+};
+
+inline DtArg host_dt(double dt) { DtArg a; a.dt = dt; a.radius = nullptr; a.cfl = 0.0; a.scale = 0.0; return a; }
+
 struct P3 {
    int32_t  c[3];
    uint32_t o;        // x cx + y cy + z cz

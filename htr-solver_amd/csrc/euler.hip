@@ -303,40 +303,6 @@ struct Euler {
       put_rec(rec, cap, w, R, Lam1, Lam, LamN);
    }
 
-   // pass C: back-projection (rhs.inl:708) and the positivity test of the
-   // limiter (rhs.inl:710-720); returns the decision word without the
-   // limiter bit, *limit says whether the face needs the limiter blend
-   __device__ uint32_t face_finish(double *Flux, const double *__restrict__ rec, int64_t cap, int64_t w,
-                                   const double *__restrict__ Fc, const uint8_t *__restrict__ bits,
-                                   const P3 &p, int t, double RK0, double RK1, double dt, bool *limit) const
-   {
-      Roe R;
-      get_rec(rec, cap, w, R);
-      const double dedp = 1.0/R.dpde;
-      const double ERoe = R.e + 0.5*mod2(R.v);
-      double sumQsp = 0.0;
-      uint32_t h = 0;
-#pragma unroll
-      for (int i = 0; i < NEQ; i++) {
-         add_column(Flux, sumQsp, i, Fc[i*cap + w], R, ERoe, dedp);
-         const uint32_t b = bits[i*cap + w];
-         h = fold_bits(fold_bits(h, b & 15u), b >> 4);
-      }
-      *limit = false;
-      if (NS > 1) {
-         const P3 pP1 = wplus(g, DIR, p, c_st.Cp[t][2]);
-         const double lim_f = dt*(ld<double>(f.m_e[DIR], p) + ld<double>(f.m_e[DIR], pP1));
-#pragma unroll
-         for (int i = 0; i < NS; i++) {
-            const double dRhoi = lim_f*Flux[i];
-            if (((RK0*ld<double>(f.Conserved_old, p, i) + RK1*cons(p, i) + dRhoi) < -1e-8) ||
-                ((RK0*ld<double>(f.Conserved_old, pP1, i) + RK1*cons(pP1, i) - dRhoi) < -1e-8))
-               *limit = true;
-         }
-      }
-      return h << 2;
-   }
-
    // pass D, on the faces the test flagged: the first-order flux
    // -0.5*(FluxP(p) + FluxM(p+1)) back-projected and blended with the
    // high-order Flux (rhs.inl:721-758)
@@ -942,12 +908,13 @@ struct TenoStage {
 template<class MX, int DIR, int SCHEME, bool LISTED>
 __global__ void __launch_bounds__(64*TENO_WAVES)
 euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv, TenoBufs b,
-                  double RK0, double RK1, double dt, double *__restrict__ F, int64_t nfaces,
+                  double RK0, double RK1, DtArg dta, double *__restrict__ F, int64_t nfaces,
                   const int32_t *__restrict__ list, const int *__restrict__ counter, htr_view dec,
                   int32_t *__restrict__ llist, int *__restrict__ lcount)
 {
    using E_t = Euler<MX, DIR>;
    using St = TenoStage<MX>;
+   const double dt = dta.get();
    constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = St::NPV;
    constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
    constexpr int iN = E_t::iN;
@@ -1116,7 +1083,7 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
 template<class MX, int DIR, bool LISTED>
 __global__ void __launch_bounds__(256)
 euler_limit_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, const htr_view hv,
-                   double RK0, double RK1, double dt, TenoBufs b, double *__restrict__ F, int64_t nfaces,
+                   double RK0, double RK1, DtArg dta, TenoBufs b, double *__restrict__ F, int64_t nfaces,
                    const int32_t *__restrict__ list, const int32_t *__restrict__ llist,
                    const int *__restrict__ lcount, htr_view dec)
 {
@@ -1130,7 +1097,7 @@ euler_limit_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
       double Flux[MX::NEQ];
 #pragma unroll
       for (int i = 0; i < MX::NEQ; i++) Flux[i] = face_ld<MX::NEQ>(F, fid, i);
-      E.face_limit(Flux, b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q), RK0, RK1, dt);
+      E.face_limit(Flux, b.rec, b.cap, w, q, ld<int>(f.nType[DIR], q), RK0, RK1, dta.get());
       store_face<MX::NEQ>(F, nfaces, fid, Flux);
       if (dec.ptr) at<uint32_t>(dec, q) |= 2u;
    }
@@ -1159,7 +1126,7 @@ face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
 static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const htr_view &hv, double RK0,
-                       double RK1, double dt, Workspace &w, int64_t nfaces, const htr_view &dec, hipStream_t s)
+                       double RK1, const DtArg &dt, Workspace &w, int64_t nfaces, const htr_view &dec, hipStream_t s)
 {
    constexpr int kMaxBlocks = 256*16;
    const TenoBufs b = teno_bufs<MX>(w.teno, nfaces);
@@ -1181,7 +1148,7 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
 
 template<class MX, int DIR, int SCHEME>
 static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
-                      double dt, const htr_view &dec, hipStream_t s)
+                      const DtArg &dt, const htr_view &dec, hipStream_t s)
 {
    const int64_t ncells = exec_cells(g);
    if (ncells == 0) return HTR_OK;
@@ -1247,9 +1214,32 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    }
 }
 
+// Workspace sizes of the Euler sweeps (every scheme, every direction) and
+// of the diffusion sweeps (heat flux per face) over the rect of g.
+template<class MX>
+WorkspaceNeed euler_workspace_need(const Geo &g)
+{
+   WorkspaceNeed n;
+   auto dir = [&](int64_t nfaces) {
+      n.faces = std::max(n.faces, nfaces*MX::NEQ);
+      n.list = std::max(n.list, 2*nfaces);
+      n.teno = std::max(n.teno, teno_doubles<MX>(nfaces));
+   };
+   if (exec_cells(g) == 0) return n;
+   dir(Faces<0>::count(g));
+   dir(Faces<1>::count(g));
+   dir(Faces<2>::count(g));
+   n.hbuf = ((int64_t)g.bsz[0] - 1)*g.cb[0] + ((int64_t)g.bsz[1] - 1)*g.cb[1] + ((int64_t)g.bsz[2] - 1)*g.cb[2] + 1;
+   return n;
+}
+
+template WorkspaceNeed euler_workspace_need<MixCP>(const Geo &);
+template WorkspaceNeed euler_workspace_need<MixAir>(const Geo &);
+template WorkspaceNeed euler_workspace_need<Mix30>(const Geo &);
+
 template<class MX>
 int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
-                   double RK0, double RK1, double dt, const htr_view &dec, hipStream_t s)
+                   double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s)
 {
 #define HTR_E_CASE(D, S) if (dir == D && scheme == S) return launch_dir<MX, D, S>(f, mix, g, RK0, RK1, dt, dec, s);
 #define HTR_E_DIRS(S) HTR_E_CASE(0, S) HTR_E_CASE(1, S) HTR_E_CASE(2, S)
@@ -1263,7 +1253,7 @@ int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, co
    return set_error(HTR_E_ARG, "euler: bad direction or scheme");
 }
 
-template int euler_dispatch<MixCP>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, double, const htr_view &, hipStream_t);
-template int euler_dispatch<MixAir>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, double, const htr_view &, hipStream_t);
+template int euler_dispatch<MixCP>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, const DtArg &, const htr_view &, hipStream_t);
+template int euler_dispatch<MixAir>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, const DtArg &, const htr_view &, hipStream_t);
 
 } // namespace htr

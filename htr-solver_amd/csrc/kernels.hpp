@@ -2,6 +2,7 @@
 // kernel translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include "common.hpp"
 
 namespace htr {
@@ -18,25 +19,29 @@ int  check_launch(const char *what);
 // buffers or counters.  Made at the first call on a stream; the face buffers
 // grow (only) when a call needs more.
 struct Workspace {
-   hipStream_t stream;
-   double  *partial;       // per-block partial maxima, capacity kPartial
-   double  *result_host;   // pinned, 1 double
-   int     *counter;       // device ints: [0] failure counter, [1] face-list length
-   double  *faces;         // face fluxes of one sweep, capacity faces_cap doubles
-   int32_t *list;          // compacted face indices, capacity list_cap
-   double  *teno;          // TENO face records, components and bits, capacity teno_cap doubles
-   double  *hbuf;          // per-cell total enthalpy of the Euler sweeps, capacity hbuf_cap doubles
-   int64_t  faces_cap, list_cap, teno_cap, hbuf_cap;
+   hipStream_t stream = nullptr;
+   double  *partial = nullptr;      // per-block partial maxima, capacity kPartial
+   double  *result_host = nullptr;  // pinned, 1 double
+   int     *counter = nullptr;      // device ints: [0] failure counter, [1] face-list length
+   double  *faces = nullptr;        // face fluxes of one sweep, capacity faces_cap doubles
+   int32_t *list = nullptr;         // compacted face indices, capacity list_cap
+   double  *teno = nullptr;         // TENO face records, capacity teno_cap doubles
+   double  *hbuf = nullptr;         // per-cell total enthalpy of the Euler sweeps, capacity hbuf_cap doubles
+   int64_t  faces_cap = 0, list_cap = 0, teno_cap = 0, hbuf_cap = 0;
+   std::mutex mu;                   // guards growth of this workspace's buffers
    static constexpr int64_t kPartial = 1 << 16;
 };
 Workspace *workspace(hipStream_t s);   // null on allocation failure (error set)
 // grows the face buffers (grow-only; an allocation happens only when a call
-// needs more than any earlier call on this device)
+// needs more than any earlier call on this stream, or than a reservation)
 int workspace_faces(Workspace &w, int64_t ndoubles, int64_t nlist, int64_t nteno);
 int workspace_enthalpy(Workspace &w, int64_t ndoubles);
+// buffer sizes a rect needs (maximum over directions and schemes)
+struct WorkspaceNeed { int64_t faces = 0, list = 0, teno = 0, hbuf = 0; };
+template<class MX> WorkspaceNeed euler_workspace_need(const Geo &g);
 
 template<class MX> int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
-                                      double RK0, double RK1, double dt, const htr_view &dec, hipStream_t s);
+                                      double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s);
 template<class MX> int diffusion_dispatch(int dir, const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int properties_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int conserved_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
@@ -51,6 +56,8 @@ template<class MX> int spectral_radius_launch(const htr_fields &f, const DMix *m
 int chem_set_launch(int block, int blocks_per_cu);
 int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, double dt, hipStream_t s);
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);
+template<class MX> int rk_stage_launch(const htr_fields &f, const DMix *mix, const Geo &g, int stage, const DtArg &dt,
+                                       const htr_view &flags, hipStream_t s);
 int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);
 int zero_counters(int *counter, hipStream_t s);   // clears Workspace::counter[0..3]
 

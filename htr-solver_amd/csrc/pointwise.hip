@@ -20,17 +20,18 @@ namespace htr {
    const P3 p = exec_point(g, tid);
 
 // ------------------------------------------------------------------ variables
+// Per-cell failure flags (the optional htr_diag.flags view)
+constexpr uint32_t FLAG_NAN = 1u;       // NaN in p, T or Conserved (DetectNaN, src/prometeo.rg:361-376)
+constexpr uint32_t FLAG_NEWTON = 2u;    // temperature Newton stopped at its iteration cap
+constexpr uint32_t FLAG_NEG_RHOY = 4u;  // a partial density was negative before the clipping of Y
+
+// UpdatePropertiesFromPrimitive of one cell (variables.hpp:62-111) from P, T, X
 template<class MX>
-__global__ void __launch_bounds__(256)
-properties_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+__device__ __forceinline__ void properties_to(const htr_fields &f, const DMix &m, const P3 &p, double P, double T,
+                                              const double *X)
 {
-   HTR_POINT_LOOP(g, n)
    constexpr int NS = MX::NS;
-   const DMix &m = *mix;
-   double X[NS], Y[NS], D[NS];
-#pragma unroll
-   for (int i = 0; i < NS; i++) X[i] = ld<double>(f.MolarFracs, p, i);
-   const double P = ld<double>(f.pressure, p), T = ld<double>(f.temperature, p);
+   double Y[NS], D[NS];
    const double MixW = MX::MolarWeightFromX(m, X);
    at<double>(f.rho, p) = MX::Rho(m, P, T, MixW);
    at<double>(f.mu, p) = MX::Viscosity(m, T, X);
@@ -41,6 +42,66 @@ properties_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    for (int i = 0; i < NS; i++) { at<double>(f.Di, p, i) = D[i]; at<double>(f.MassFracs, p, i) = Y[i]; }
    const double gamma = MX::Gamma(m, T, MixW, Y);
    at<double>(f.SoS, p) = MX::SpeedOfSound(m, T, gamma, MixW);
+}
+
+template<class MX>
+struct CellPrim { double T, P, X[MX::NS], v[3]; };
+
+// UpdatePrimitiveFromConserved of one cell (variables.hpp:160-192) from the
+// conserved vector q and the temperature guess; returns failure flags
+template<class MX>
+__device__ __forceinline__ uint32_t primitive_of(const DMix &m, const double *q, double Tguess, CellPrim<MX> &o)
+{
+   constexpr int NS = MX::NS;
+   uint32_t flags = 0;
+   double Yi[NS];
+   double rho;
+   if (MX::CP) {
+      rho = q[0];
+      Yi[0] = q[0]/rho;
+   } else {
+      rho = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) { rho += q[i]; if (q[i] < 0.0) flags |= FLAG_NEG_RHOY; }
+      const double irho = 1.0/rho;
+#pragma unroll
+      for (int i = 0; i < NS; i++) { Yi[i] = irho*q[i]; Yi[i] = dmax(Yi[i], 1e-60); Yi[i] = dmin(Yi[i], 1.0); }
+   }
+   const double MixW = MX::MolarWeightFromY(m, Yi);
+   MX::MolarFractions(m, o.X, MixW, Yi);
+   const double rhoInv = 1.0/rho;
+#pragma unroll
+   for (int i = 0; i < 3; i++) o.v[i] = q[NS+i]*rhoInv;
+   double k2 = 0.0; k2 += o.v[0]*o.v[0]; k2 += o.v[1]*o.v[1]; k2 += o.v[2]*o.v[2];
+   const double e = q[NS+3]*rhoInv - 0.5*k2;
+   int iters = 0;
+   o.T = MX::TFromInternalEnergy(m, e, Tguess, Yi, &iters);
+   if (!MX::CP && iters >= 1000) flags |= FLAG_NEWTON;
+   o.P = MX::PFromRhoAndT(m, rho, MixW, o.T);
+   return flags;
+}
+
+template<class MX>
+__device__ __forceinline__ void store_primitive(const htr_fields &f, const P3 &p, const CellPrim<MX> &o)
+{
+   at<double>(f.temperature, p) = o.T;
+   at<double>(f.pressure, p) = o.P;
+#pragma unroll
+   for (int i = 0; i < MX::NS; i++) at<double>(f.MolarFracs, p, i) = o.X[i];
+#pragma unroll
+   for (int i = 0; i < 3; i++) at<double>(f.velocity, p, i) = o.v[i];
+}
+
+template<class MX>
+__global__ void __launch_bounds__(256)
+properties_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NS = MX::NS;
+   double X[NS];
+#pragma unroll
+   for (int i = 0; i < NS; i++) X[i] = ld<double>(f.MolarFracs, p, i);
+   properties_to<MX>(f, *mix, p, ld<double>(f.pressure, p), ld<double>(f.temperature, p), X);
 }
 
 template<class MX>
@@ -65,37 +126,42 @@ __global__ void __launch_bounds__(256)
 primitive_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
 {
    HTR_POINT_LOOP(g, n)
-   constexpr int NS = MX::NS;
-   const DMix &m = *mix;
-   double rhoYi[NS], Yi[NS], X[NS], v[3];
+   double q[MX::NEQ];
 #pragma unroll
-   for (int i = 0; i < NS; i++) rhoYi[i] = ld<double>(f.Conserved, p, i);
-   double rho;
-   if (MX::CP) {
-      rho = rhoYi[0];
-      Yi[0] = rhoYi[0]/rho;
-   } else {
-      rho = 0.0;
+   for (int i = 0; i < MX::NEQ; i++) q[i] = ld<double>(f.Conserved, p, i);
+   CellPrim<MX> o;
+   primitive_of<MX>(*mix, q, ld<double>(f.temperature, p), o);
+   store_primitive<MX>(f, p, o);
+}
+
+// One SSP-RK3 stage of a cell in one pass (src/prometeo_rk.rg:88-112, then
+// UpdatePrimitiveFromConserved and UpdatePropertiesFromPrimitive of
+// UpdatePrimitivesAndBCsFromConserved, src/prometeo.rg:428-478, without the
+// boundary tasks): Conserved = c0 Conserved_old + c1 Conserved + c2
+// Conserved_t dt, then primitives and properties from the updated vector held
+// in registers -- the same operations as the three separate tasks.
+template<class MX>
+__global__ void __launch_bounds__(256)
+rk_stage_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double c0, double c1, double c2,
+                DtArg dta, htr_view flags, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NEQ = MX::NEQ;
+   const double dt = dta.get();
+   double q[NEQ];
+   bool bad = false;
 #pragma unroll
-      for (int i = 0; i < NS; i++) rho += rhoYi[i];
-      const double irho = 1.0/rho;
-#pragma unroll
-      for (int i = 0; i < NS; i++) { Yi[i] = irho*rhoYi[i]; Yi[i] = dmax(Yi[i], 1e-60); Yi[i] = dmin(Yi[i], 1.0); }
+   for (int i = 0; i < NEQ; i++) {
+      q[i] = c0*ld<double>(f.Conserved_old, p, i) + c1*ld<double>(f.Conserved, p, i) + c2*ld<double>(f.Conserved_t, p, i)*dt;
+      at<double>(f.Conserved, p, i) = q[i];
+      bad |= isnan(q[i]);
    }
-   const double MixW = MX::MolarWeightFromY(m, Yi);
-   MX::MolarFractions(m, X, MixW, Yi);
-   const double rhoInv = 1.0/rho;
-#pragma unroll
-   for (int i = 0; i < 3; i++) v[i] = ld<double>(f.Conserved, p, NS+i)*rhoInv;
-   double k2 = 0.0; k2 += v[0]*v[0]; k2 += v[1]*v[1]; k2 += v[2]*v[2];
-   const double e = ld<double>(f.Conserved, p, NS+3)*rhoInv - 0.5*k2;
-   const double T = MX::TFromInternalEnergy(m, e, ld<double>(f.temperature, p), Yi);
-   at<double>(f.temperature, p) = T;
-   at<double>(f.pressure, p) = MX::PFromRhoAndT(m, rho, MixW, T);
-#pragma unroll
-   for (int i = 0; i < NS; i++) at<double>(f.MolarFracs, p, i) = X[i];
-#pragma unroll
-   for (int i = 0; i < 3; i++) at<double>(f.velocity, p, i) = v[i];
+   CellPrim<MX> o;
+   uint32_t fl = primitive_of<MX>(*mix, q, ld<double>(f.temperature, p), o);
+   store_primitive<MX>(f, p, o);
+   properties_to<MX>(f, *mix, p, o.P, o.T, o.X);
+   if (bad || isnan(o.P) || isnan(o.T)) fl |= FLAG_NAN;
+   if (flags.ptr) at<uint32_t>(flags, p) = fl;
 }
 
 // -------------------------------------------------------------------- sensors
@@ -318,6 +384,15 @@ int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, doub
    HTR_LAUNCH_POINT(rk_update_kernel, f, g, nEq, C[stage][0], C[stage][1], C[stage][2], dt)
 }
 
+template<class MX>
+int rk_stage_launch(const htr_fields &f, const DMix *mix, const Geo &g, int stage, const DtArg &dt,
+                    const htr_view &flags, hipStream_t s)
+{
+   static const double C[3][3] = { { 1.0, 0.0, 1.0 }, { 3.0/4.0, 1.0/4.0, 1.0/4.0 }, { 1.0/3.0, 2.0/3.0, 2.0/3.0 } };
+   if (stage < 0 || stage > 2) return set_error(HTR_E_ARG, "rk: bad stage %d", stage);
+   HTR_LAUNCH_POINT(rk_stage_kernel<MX>, f, mix, g, C[stage][0], C[stage][1], C[stage][2], dt, flags)
+}
+
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
 { HTR_LAUNCH_POINT(rk_init_kernel, f, g, nEq) }
 
@@ -351,7 +426,9 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    template int primitive_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);   \
    template int chem_source_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t); \
    template int shock_sensor_launch<MX>(int, const htr_fields &, const Geo &, hipStream_t);         \
-   template int spectral_radius_launch<MX>(const htr_fields &, const DMix *, const Geo &, double *, hipStream_t);
+   template int spectral_radius_launch<MX>(const htr_fields &, const DMix *, const Geo &, double *, hipStream_t); \
+   template int rk_stage_launch<MX>(const htr_fields &, const DMix *, const Geo &, int, const DtArg &,           \
+                                    const htr_view &, hipStream_t);
 HTR_INST_POINT(MixCP)
 HTR_INST_POINT(MixAir)
 HTR_INST_POINT(Mix30)

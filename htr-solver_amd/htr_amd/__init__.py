@@ -45,7 +45,7 @@ class Fields(C.Structure):
 
 
 class Diag(C.Structure):
-    _fields_ = [("decisions", View * 3), ("nsteps", View)]
+    _fields_ = [("decisions", View * 3), ("nsteps", View), ("flags", View)]
 
 
 EXPORTS = [
@@ -57,6 +57,7 @@ EXPORTS = [
     "htr_chem_source", "htr_chem_update", "htr_max_spectral_radius",
     "htr_rk_update", "htr_rk_init", "htr_full_rhs",
     "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
+    "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl",
 ]
 
 _lib = None
@@ -83,6 +84,9 @@ def _declare(L):
         "htr_profile": (i, [i]), "htr_profile_report": (i, [C.c_char_p, C.c_size_t]),
         "htr_chem_launch_config": (i, [i, i]),
         "htr_stencil_table": (i, [i, i, i, dp]),
+        "htr_workspace_reserve": (i, [G, F, vp, vp]),
+        "htr_rk_stage": (i, [G, F, i, d, d, vp, vp, D, vp]),
+        "htr_full_rhs_cfl": (i, [G, F, i, d, d, d, d, vp, i, vp, vp]),
         "htr_mix_create_constprop": (i, [C.POINTER(vp), d, d, d, i, d, d, d]),
         "htr_mix_create_airmix": (i, [C.POINTER(vp), d, d, d, i, ip, dp]),
         "htr_mix_create_species": (i, [C.POINTER(vp), i, dp, fp, ip, dp, dp, dp, ip, dp]),

@@ -21,7 +21,7 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "update_using_skew_symmetric_euler_flux", "update_using_diffusion_flux",
            "add_chemistry_sources", "update_chemistry", "calculate_max_spectral_radius",
            "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report",
-           "chem_launch_config", "chem_autotune"]
+           "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve"]
 
 
 def stream_ptr(stream=None):
@@ -125,7 +125,7 @@ class FieldSet:
     and presented through a permuted view."""
 
     def __init__(self, n, nspec, device="cuda", bounds_lo=(0, 0, 0), decisions=False, nsteps=False,
-                 layout="soa"):
+                 layout="soa", flags=False):
         nx, ny, nz = n
         self.n = tuple(n)
         self.ns = nspec
@@ -149,6 +149,8 @@ class FieldSet:
                 self.t["decisions%d" % d] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
         if nsteps:
             self.t["nsteps"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+        if flags:
+            self.t["flags"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
         self._c = None
 
     def __getitem__(self, k):
@@ -192,6 +194,8 @@ class FieldSet:
                 dg.decisions[d] = self.view("decisions%d" % d)
         if "nsteps" in self.t:
             dg.nsteps = self.view("nsteps")
+        if "flags" in self.t:
+            dg.flags = self.view("flags")
         return dg
 
     def grid(self, lo=(0, 0, 0), hi=None):
@@ -272,6 +276,28 @@ def rk_init(fs, nEq, grid=None, stream=None):
 def full_rhs(fs, scheme, RK_coeffs, deltaTime, chem_source, mix, grid=None, stream=None):
     check(lib().htr_full_rhs(_g(fs, grid), C.byref(fs.c()), scheme, RK_coeffs[0], RK_coeffs[1], deltaTime,
                              int(chem_source), mix.h, stream_ptr(stream)))
+
+
+def full_rhs_cfl(fs, scheme, RK_coeffs, cfl, scale, radius, chem_source, mix, grid=None, stream=None):
+    """full_rhs with the stage's deltaTime = (cfl / radius) * scale formed on
+    the device; radius is a one-element float64 device tensor."""
+    check(lib().htr_full_rhs_cfl(_g(fs, grid), C.byref(fs.c()), scheme, RK_coeffs[0], RK_coeffs[1], cfl, scale,
+                                 C.c_void_p(radius.data_ptr()), int(chem_source), mix.h, stream_ptr(stream)))
+
+
+def rk_stage(fs, stage, mix, deltaTime=0.0, cfl=0.0, radius=None, grid=None, stream=None, diag=False):
+    """Fused SSP-RK3 stage: update, primitive recompute and properties in one
+    pass per cell; with radius (device tensor) the step is cfl / radius."""
+    dg = C.byref(fs.diag()) if diag else None
+    rp = C.c_void_p(radius.data_ptr()) if radius is not None else None
+    check(lib().htr_rk_stage(_g(fs, grid), C.byref(fs.c()), stage, deltaTime, cfl, rp, mix.h, dg,
+                             stream_ptr(stream)))
+
+
+def workspace_reserve(fs, mix, grid=None, stream=None):
+    """Sizes the stream's library workspace for this tile (no allocation in
+    later calls on it)."""
+    check(lib().htr_workspace_reserve(_g(fs, grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
 
 
 def profile(enable=True):

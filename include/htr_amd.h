@@ -26,10 +26,12 @@
  *     (the reference returns a future), and htr_chem_update, which reads its
  *     failure count back;
  *   - device workspace: one per (device, stream), created at the first call
- *     on that stream; the Euler TENO face buffers grow (only) at the first
- *     call on a larger rect, so a solver's first step allocates and later
- *     steps do not (a growth waits for the stream's earlier work before it
- *     frees the old buffer); calls on different streams may overlap;
+ *     on that stream; the Euler and diffusion face buffers grow (only) at the
+ *     first call on a larger rect, so a solver's first step allocates and
+ *     later steps do not (a growth waits for the stream's earlier work before
+ *     it frees the old buffer: a synchronisation point);
+ *     htr_workspace_reserve sizes them up front; calls on different streams
+ *     may overlap;
  *   - every function returns 0 on success or a negative HTR_E_* code;
  *     htr_last_error() describes the last failure of the calling thread;
  *   - mixtures are uploaded once (htr_mix_*), as LoadMixtureTask does.
@@ -98,10 +100,18 @@ typedef struct {
  *   decisions[d]: uint32 per cell, the decision word of the Euler face on the
  *                 cell's + side (bit0 KG used, bit1 limiter fired, bits 2..31
  *                 fold of the TENO stencil-selection bits);
- *   nsteps:       int32 per cell, Rosenbrock steps (negative: cap reached). */
+ *   nsteps:       int32 per cell, Rosenbrock steps (negative: cap reached);
+ *   flags:        uint32 per cell, failures of the primitive recompute
+ *                 (htr_rk_stage): bit0 NaN in pressure, temperature or
+ *                 Conserved (the reference's DetectNaN, src/prometeo.rg:361-376),
+ *                 bit1 the temperature Newton iteration hit its cap of 1000
+ *                 (src/Mixtures/MultiComponent.inl:189-209, which returns
+ *                 silently), bit2 a partial density was negative before Y was
+ *                 clipped (src/Mixtures/MultiComponent.inl:59-66). */
 typedef struct {
    htr_view decisions[3];
    htr_view nsteps;
+   htr_view flags;
 } htr_diag;
 
 typedef struct htr_mix htr_mix;
@@ -210,6 +220,26 @@ int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mi
 int htr_rk_update(const htr_grid *g, const htr_fields *f, int nEq, int stage, double dt, void *stream);
 /* Conserved_old = Conserved (InitializeTemporaries) */
 int htr_rk_init(const htr_grid *g, const htr_fields *f, int nEq, void *stream);
+/* One SSP-RK3 stage fused in one pass per cell: the update of htr_rk_update
+ * (UpdateVars, src/prometeo_rk.rg:88-112), then UpdatePrimitiveFromConserved
+ * and UpdatePropertiesFromPrimitive (the interior part of
+ * UpdatePrimitivesAndBCsFromConserved, src/prometeo.rg:428-478) from the
+ * updated vector, with the same operations as the three tasks.  The step is
+ * deltaTime, or, when d_radius is not null, cfl / *d_radius read on the
+ * device (ConstantCFL, src/prometeo.rg:1159-1165) so that a step loop needs
+ * no host read-back.  diag->flags (optional) receives the failure bits. */
+int htr_rk_stage(const htr_grid *g, const htr_fields *f, int stage, double deltaTime, double cfl,
+                 const double *d_radius, const htr_mix *m, const htr_diag *diag, void *stream);
+
+/* ---- workspace (no reference counterpart) --------------------------------- */
+/* Sizes the stream's workspace for every Euler scheme and direction and the
+ * diffusion sweeps on rects no larger than g's execution rect with fields laid
+ * out as f (only the views' strides are read) and mixture m.  Later calls on
+ * this stream over such rects allocate nothing and never synchronise; a call
+ * over a larger rect grows the buffers, waiting for the stream's earlier work
+ * before it frees the old ones (a synchronisation point).  Set-up code and
+ * request admission call it; it allocates and may synchronise the stream. */
+int htr_workspace_reserve(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
 
 /* ---- one full right-hand side of an RK stage ------------------------------ */
 /* The sequence of src/prometeo.rg:503-578 on one tile: Conserved_t = 0,
@@ -217,6 +247,13 @@ int htr_rk_init(const htr_grid *g, const htr_fields *f, int nEq, void *stream);
  * same accumulation order as the reference tasks. */
 int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
                  double deltaTime, int chem_source, const htr_mix *m, void *stream);
+/* The same with the stage's deltaTime formed on the device as
+ * (cfl / *d_radius) * scale, i.e. Integrator_deltaTime*RK_C[STAGE][3] of a
+ * ConstantCFL step (src/prometeo.rg:1165, :1419), d_radius being a device
+ * pointer to the maximum spectral radius. */
+int htr_full_rhs_cfl(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
+                     double cfl, double scale, const double *d_radius, int chem_source, const htr_mix *m,
+                     void *stream);
 
 #ifdef __cplusplus
 }

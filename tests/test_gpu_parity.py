@@ -485,3 +485,61 @@ def test_wall_bounded_node_types(parity_record):
         e = norm_err(host(fs["Conserved_t"]), f["Conserved_t"])
         say("  wall diffusion dir %d: %.3e" % (d, max(e)))
         assert max(e) <= TOL
+
+
+def test_fused_rk_stage_and_device_dt(tgv16, air16):
+    """The fused RK stage (update + primitives + properties in one pass) is
+    bit-identical to the three separate tasks, and a full RHS whose deltaTime
+    is formed on the device from a radius in device memory equals the host
+    deltaTime call.  Failure flags stay clear on these healthy states."""
+    import htr_amd as H
+    for name, case, nEq in (("tgv16", tgv16, 5), ("air16", air16, 9)):
+        st, meta, n, om = case
+        say("case %s fused RK stage / device dt" % name)
+        gm = gpu_mix(meta)
+        r = H.calculate_max_spectral_radius(to_gpu(st, om.nSpec, n), gm)
+        radius = torch.tensor([r], dtype=torch.float64, device="cuda")
+        cfl = 0.4
+        dt = cfl / r
+        a = to_gpu(st, om.nSpec, n, flags=True)
+        b = to_gpu(st, om.nSpec, n)
+        for fs in (a, b):
+            H.rk_init(fs, nEq)
+        for stage, c3 in enumerate((1.0, 0.25, 2.0 / 3.0)):
+            RK = [(1.0, 0.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)][stage]
+            H.full_rhs_cfl(a, 0, RK, cfl, c3, radius, True, gm)
+            H.full_rhs(b, 0, RK, dt * c3, True, gm)
+            assert float((a["Conserved_t"] - b["Conserved_t"]).abs().max()) == 0.0
+            H.rk_stage(a, stage, gm, cfl=cfl, radius=radius, diag=True)
+            H.rk_update(b, nEq, stage, dt)
+            H.update_primitive_from_conserved(b, gm)
+            H.update_properties_from_primitive(b, gm)
+            for k in ("Conserved", "temperature", "pressure", "MolarFracs", "velocity", "rho", "mu", "lam", "Di",
+                      "SoS", "MassFracs"):
+                d = float((a[k] - b[k]).abs().max())
+                assert d == 0.0, (name, stage, k, d)
+            assert int(a["flags"].abs().max()) == 0
+        say("  fused stage == separate tasks, device dt == host dt (bit-identical, 3 stages)")
+
+
+def test_failure_flags():
+    """A NaN in the conserved vector and a negative partial density are
+    reported per cell by the fused stage."""
+    import htr_amd as H
+    st, meta = air_state(8, seed=1)
+    n = (8, 8, 8)
+    st, om = complete_state(st, meta, n, sensors=False)
+    fs = to_gpu(st, 5, n, flags=True)
+    gm = gpu_mix(meta)
+    H.rk_init(fs, 9)
+    fs["Conserved_t"] = 0.0
+    fs["Conserved_old"][0, 1, 2, 3] = float("nan")
+    fs["Conserved_old"][2, 4, 4, 4] = -1e-3 * abs(float(fs["Conserved_old"][2, 4, 4, 4])) - 1e-12
+    H.rk_stage(fs, 0, gm, deltaTime=1e-3, diag=True)
+    fl = host(fs["flags"])
+    say("  flags: nan %d neg %d" % (int(np.sum(fl & 1)), int(np.sum(fl & 4))))
+    assert fl[1, 2, 3] & 1 and fl[4, 4, 4] & 4
+    others = fl.copy()
+    others[1, 2, 3] = 0
+    others[4, 4, 4] = 0
+    assert np.all(others == 0)
