@@ -770,18 +770,21 @@ static int full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double R
    if (scheme != HTR_EULER_KG) { HTR_NEED(SoS); if (m->host.nSpec > 1) HTR_NEED(Conserved_old); }
    if (scheme == HTR_EULER_HYBRID) for (int d = 0; d < 3; d++) HTR_NEED(shockSensor[d]);
    hipStream_t s = (hipStream_t)stream;
-   rc = zero_rhs_launch(fr, geo, m->host.nSpec + 4, s);
-   if (rc) return rc;
+   const MixKind k = kind_of(m);
+   if (k != K_CP && k != K_AIR) return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
    if (chem_source && !m->host.constprop) {
-      rc = htr_chem_source(g, f, m, stream);
-      if (rc) return rc;
+      HTR_NEED(pressure); HTR_NEED(temperature);
+      rc = chem_source_launch<MixAir>(fr, m->dev, geo, s, true);     // zeroes Conserved_t as it goes
+   } else {
+      rc = zero_rhs_launch(fr, geo, m->host.nSpec + 4, s);
    }
+   if (rc) return rc;
+   // total enthalpy once for the three sweeps
+   rc = (k == K_CP) ? enthalpy_launch<MixCP>(fr, geo, s) : enthalpy_launch<MixAir>(fr, geo, s);
+   if (rc) return rc;
    for (int d = 2; d >= 0; d--) {
-      switch (kind_of(m)) {
-         case K_CP:  rc = euler_dispatch<MixCP>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s); break;
-         case K_AIR: rc = euler_dispatch<MixAir>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s); break;
-         default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
-      }
+      rc = (k == K_CP) ? euler_dispatch<MixCP>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s, true)
+                       : euler_dispatch<MixAir>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s, true);
       if (rc) return rc;
    }
    for (int d = 2; d >= 0; d--) {

@@ -1148,7 +1148,7 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
 
 template<class MX, int DIR, int SCHEME>
 static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
-                      const DtArg &dt, const htr_view &dec, hipStream_t s)
+                      const DtArg &dt, const htr_view &dec, hipStream_t s, bool enthalpy_ready)
 {
    const int64_t ncells = exec_cells(g);
    if (ncells == 0) return HTR_OK;
@@ -1168,8 +1168,10 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    htr_view hv = {};
    hv.ptr = w.hbuf;
    hv.sx = 8;                                  // one double per base element (k = 1)
-   HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
-   if ((rc = check_launch("enthalpy_kernel"))) return rc;
+   if (!enthalpy_ready) {
+      HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
+      if ((rc = check_launch("enthalpy_kernel"))) return rc;
+   }
    if constexpr (SCHEME != HTR_EULER_KG) {
       if ((rc = zero_counters(w.counter, s))) return rc;    // TENO list and limiter list counts
    }
@@ -1214,6 +1216,28 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    }
 }
 
+// Total enthalpy of every cell of the bounds into the stream's workspace
+// (once per RHS; the three sweeps then skip it)
+template<class MX>
+int enthalpy_launch(const htr_fields &f, const Geo &g, hipStream_t s)
+{
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   const int64_t nb_cells = (int64_t)g.bsz[0]*g.bsz[1]*g.bsz[2];
+   const int64_t span = ((int64_t)g.bsz[0] - 1)*g.cb[0] + ((int64_t)g.bsz[1] - 1)*g.cb[1] +
+                        ((int64_t)g.bsz[2] - 1)*g.cb[2] + 1;
+   int rc = workspace_enthalpy(*wp, span);
+   if (rc) return rc;
+   htr_view hv = {};
+   hv.ptr = wp->hbuf;
+   hv.sx = 8;
+   HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
+   return check_launch("enthalpy_kernel");
+}
+
+template int enthalpy_launch<MixCP>(const htr_fields &, const Geo &, hipStream_t);
+template int enthalpy_launch<MixAir>(const htr_fields &, const Geo &, hipStream_t);
+
 // Workspace sizes of the Euler sweeps (every scheme, every direction) and
 // of the diffusion sweeps (heat flux per face) over the rect of g.
 template<class MX>
@@ -1239,9 +1263,10 @@ template WorkspaceNeed euler_workspace_need<Mix30>(const Geo &);
 
 template<class MX>
 int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
-                   double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s)
+                   double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s,
+                   bool enthalpy_ready)
 {
-#define HTR_E_CASE(D, S) if (dir == D && scheme == S) return launch_dir<MX, D, S>(f, mix, g, RK0, RK1, dt, dec, s);
+#define HTR_E_CASE(D, S) if (dir == D && scheme == S) return launch_dir<MX, D, S>(f, mix, g, RK0, RK1, dt, dec, s, enthalpy_ready);
 #define HTR_E_DIRS(S) HTR_E_CASE(0, S) HTR_E_CASE(1, S) HTR_E_CASE(2, S)
    HTR_E_DIRS(HTR_EULER_HYBRID)
    HTR_E_DIRS(HTR_EULER_TENO)
@@ -1253,7 +1278,7 @@ int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, co
    return set_error(HTR_E_ARG, "euler: bad direction or scheme");
 }
 
-template int euler_dispatch<MixCP>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, const DtArg &, const htr_view &, hipStream_t);
-template int euler_dispatch<MixAir>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, const DtArg &, const htr_view &, hipStream_t);
+template int euler_dispatch<MixCP>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, const DtArg &, const htr_view &, hipStream_t, bool);
+template int euler_dispatch<MixAir>(int, int, const htr_fields &, const DMix *, const Geo &, double, double, const DtArg &, const htr_view &, hipStream_t, bool);
 
 } // namespace htr

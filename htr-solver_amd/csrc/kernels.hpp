@@ -39,16 +39,22 @@ int workspace_enthalpy(Workspace &w, int64_t ndoubles);
 // buffer sizes a rect needs (maximum over directions and schemes)
 struct WorkspaceNeed { int64_t faces = 0, list = 0, teno = 0, hbuf = 0; };
 template<class MX> WorkspaceNeed euler_workspace_need(const Geo &g);
+template<class MX> int enthalpy_launch(const htr_fields &f, const Geo &g, hipStream_t s);
 
+// enthalpy_ready: the total-enthalpy workspace array already holds (rhoE + p)/rho
+// of every cell of the bounds (formed once per RHS by the full-RHS entry)
 template<class MX> int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
-                                      double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s);
+                                      double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s,
+                                      bool enthalpy_ready = false);
 template<class MX> int diffusion_dispatch(int dir, const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int properties_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int conserved_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int primitive_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int shock_sensor_launch(int dir, const htr_fields &f, const Geo &g, hipStream_t s);
 int ducros_launch(const htr_fields &f, const Geo &g, double vorticityScale, hipStream_t s);
-template<class MX> int chem_source_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
+// zero: also clear Conserved_t first (the source is the stage's first term)
+template<class MX> int chem_source_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s,
+                                          bool zero = false);
 template<class MX> int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, double dt,
                                           int precision, const htr_view &nsteps, hipStream_t s);
 template<class MX> int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g,

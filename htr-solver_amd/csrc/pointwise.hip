@@ -226,7 +226,7 @@ shock_sensor_kernel(const htr_fields f, const Geo g, int64_t n)
 // ------------------------------------------------------------------ chemistry
 template<class MX>
 __global__ void __launch_bounds__(256)
-chem_source_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int64_t n)
+chem_source_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int zero, int64_t n)
 {
    HTR_POINT_LOOP(g, n)
    constexpr int NS = MX::NS;
@@ -234,8 +234,17 @@ chem_source_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
 #pragma unroll
    for (int i = 0; i < NS; i++) Y[i] = ld<double>(f.MassFracs, p, i);
    MX::ProductionRates(*mix, w, ld<double>(f.rho, p), ld<double>(f.pressure, p), ld<double>(f.temperature, p), Y);
+   if (zero) {
+      // first term of the stage's RHS: Conserved_t = 0 + source, the other
+      // components cleared (the zeroing of InitializeTimeDerivatives fused)
 #pragma unroll
-   for (int i = 0; i < NS; i++) at<double>(f.Conserved_t, p, i) += w[i];
+      for (int i = 0; i < NS; i++) at<double>(f.Conserved_t, p, i) = 0.0 + w[i];
+#pragma unroll
+      for (int i = NS; i < MX::NEQ; i++) at<double>(f.Conserved_t, p, i) = 0.0;
+   } else {
+#pragma unroll
+      for (int i = 0; i < NS; i++) at<double>(f.Conserved_t, p, i) += w[i];
+   }
 }
 
 // ----------------------------------------------------------- spectral radius
@@ -360,8 +369,8 @@ template<class MX> int conserved_launch(const htr_fields &f, const DMix *mix, co
 { HTR_LAUNCH_POINT(conserved_kernel<MX>, f, mix, g) }
 template<class MX> int primitive_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
 { HTR_LAUNCH_POINT(primitive_kernel<MX>, f, mix, g) }
-template<class MX> int chem_source_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s)
-{ HTR_LAUNCH_POINT(chem_source_kernel<MX>, f, mix, g) }
+template<class MX> int chem_source_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s, bool zero)
+{ HTR_LAUNCH_POINT(chem_source_kernel<MX>, f, mix, g, zero ? 1 : 0) }
 
 int ducros_launch(const htr_fields &f, const Geo &g, double vorticityScale, hipStream_t s)
 {
@@ -424,7 +433,7 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    template int properties_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);  \
    template int conserved_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);   \
    template int primitive_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);   \
-   template int chem_source_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t); \
+   template int chem_source_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t, bool); \
    template int shock_sensor_launch<MX>(int, const htr_fields &, const Geo &, hipStream_t);         \
    template int spectral_radius_launch<MX>(const htr_fields &, const DMix *, const Geo &, double *, hipStream_t); \
    template int rk_stage_launch<MX>(const htr_fields &, const DMix *, const Geo &, int, const DtArg &,           \
