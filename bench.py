@@ -270,11 +270,11 @@ def chemistry(n, dt, seed, cpu, threads, tune=True):
     for prec, name in ((H.HTR_CHEM_FP64, "fp64"), (H.HTR_CHEM_FP32_IR, "fp32_ir")):
         if tune:
             best, table = H.chem_autotune(fs, mix, dt, prec, grid=sub,
-                                          configs=[(64, 4), (64, 16), (128, 8), (256, 4)])
+                                          configs=[(16, 0), (32, 0), (64, 0), (64, 4)])
             print("chemistry %s: autotune %s -> %s" % (name, {k: round(v, 2) for k, v in table.items()}, best),
                   file=sys.stderr, flush=True)
         else:
-            best, table = (64, 16), {}
+            best, table = (64, 0), {}
             H.chem_launch_config(*best)
         for k in init:
             fs[k].copy_(init[k])
@@ -289,7 +289,7 @@ def chemistry(n, dt, seed, cpu, threads, tune=True):
         steps = fs["nsteps"].cpu().numpy().ravel()
         res[name] = fs["Conserved"][:30].cpu().numpy().copy()
         print("chemistry %s: %d cells in %.1f ms" % (name, n ** 3, ms), file=sys.stderr, flush=True)
-        out[name] = {"solves_per_s": n ** 3 / (ms * 1e-3), "ms": ms, "launch": {"block": best[0], "blocks_per_cu": best[1]},
+        out[name] = {"solves_per_s": n ** 3 / (ms * 1e-3), "ms": ms, "launch": {"lanes_per_cell": best[0], "blocks_per_cu": best[1]},
                      "autotune_ms": {"%dx%d" % k: round(v, 3) for k, v in table.items()},
                      "steps_mean": float(steps.mean()), "steps_p50": float(np.percentile(steps, 50)),
                      "steps_p99": float(np.percentile(steps, 99)), "steps_max": int(steps.max())}

@@ -236,10 +236,28 @@ static void store_refs(DMix &m, double PRef, double TRef, double LRef, const dou
    }
 }
 
+// species-major stoichiometry (DMix::spr_*), from the reaction list
+static void build_species_reactions(DMix &d)
+{
+   int n = 0;
+   for (int s = 0; s < d.nSpec; s++) {
+      d.spr_ptr[s] = n;
+      for (int k = 0; k < d.nReac; k++) {
+         const DReac &r = d.rc[k];
+         for (int i = 0; i < r.Ne; i++)
+            if (r.ed_ind[i] == s) { d.spr_reac[n] = (int16_t)k; d.spr_nu[n] = -r.ed_nu[i]; n++; }
+         for (int i = 0; i < r.Np; i++)
+            if (r.pd_ind[i] == s) { d.spr_reac[n] = (int16_t)k; d.spr_nu[n] = r.pd_nu[i]; n++; }
+      }
+   }
+   d.spr_ptr[d.nSpec] = n;
+}
+
 static int upload(htr_mix *m)
 {
    int rc = ensure_device();
    if (rc) return rc;
+   if (!m->host.constprop) build_species_reactions(m->host);
    if (!m->dev && hipMalloc(&m->dev, sizeof(DMix)) != hipSuccess)
       return set_error(HTR_E_DEVICE, "mixture allocation failed");
    if (hipMemcpy(m->dev, &m->host, sizeof(DMix), hipMemcpyHostToDevice) != hipSuccess)
@@ -698,8 +716,8 @@ int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int preci
    if (precision != HTR_CHEM_FP64 && precision != HTR_CHEM_FP32_IR) return set_error(HTR_E_ARG, "bad precision %d", precision);
    HTR_NEED(Conserved); HTR_NEED(Conserved_t); HTR_NEED(Conserved_t_old); HTR_NEED(temperature);
    switch (kind_of(m)) {
-      case K_AIR: return chem_update_launch<MixAir>(fr, m->dev, geo, dt, precision, ns, (hipStream_t)stream);
-      case K_30:  return chem_update_launch<Mix30>(fr, m->dev, geo, dt, precision, ns, (hipStream_t)stream);
+      case K_AIR: return chem_update_launch<MixAir>(fr, m->dev, geo, dt, precision, ns, m->host.nReac, (hipStream_t)stream);
+      case K_30:  return chem_update_launch<Mix30>(fr, m->dev, geo, dt, precision, ns, m->host.nReac, (hipStream_t)stream);
       default: return set_error(HTR_E_UNSUPPORTED, "implicit chemistry: no kernel instance for %d species", m->host.nSpec);
    }
 }

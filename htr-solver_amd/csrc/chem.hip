@@ -5,277 +5,472 @@
 // src/prometeo_chem.cc:36-71 (task loop), src/Utils/math_utils.hpp:87-438
 // (LUdec, Rosenbrock: FD Jacobian, step, itercheck, stepcheck).
 //
-// Two linear-solve precisions:
-//   HTR_CHEM_FP64     Crout LU with implicit scaling and partial pivoting in
-//                     FP64, as the reference;
-//   HTR_CHEM_FP32_IR  the matrix equilibrated by powers of two and factorised
-//                     in FP32; each of the four stage solves is recovered to
-//                     FP64 accuracy by residual-based iterative refinement
-//                     against the FP64 matrix (r = b - A x in FP64,
-//                     x += LU32 \ r) until the correction is below 1e-15 of
-//                     the solution; if a sweep fails to halve the correction
-//                     the solve falls back to an FP64 LU of the same matrix
-//                     (made once per factorisation), so the result always has
-//                     FP64 accuracy.
+// A group of L lanes of one wave integrates one cell (64/L cells per wave);
+// the groups pull cells from an atomic work queue, so cells with many steps
+// do not hold others back.  The cell's Jacobian, its LU factors and the
+// solver vectors live in LDS; the lanes share the work of every step:
+//   right-hand side   lanes over species (mass fractions, the terms of the
+//                     energy and heat-capacity sums of the temperature Newton,
+//                     concentrations, Gibbs energies, production rates) and
+//                     over reactions (rate of progress); every sum is formed
+//                     in the reference's order by each lane from the terms
+//                     in LDS, so it rounds as the reference does;
+//   FD Jacobian       2 nEq right-hand sides, the temperature guess carried
+//                     from one to the next as the reference carries it;
+//   LU                right-looking elimination, lanes over rows: every entry
+//                     receives the same products in the same order as the
+//                     reference's Crout loops, so the factors are the same;
+//                     pivot = the last row attaining the maximum scaled
+//                     magnitude, as the reference's `>=' search;
+//   solves            forward substitution column by column (same order),
+//                     back substitution row by row in the reference's order.
 //
-// One thread integrates one cell (cells of a wave march independently; the
-// step count varies per cell).  The cell loop is a grid-stride loop over a
-// persistent grid sized to the device, so long-running cells do not hold a
-// whole launch back more than one cell's worth.
+// Two linear-solve precisions:
+//   HTR_CHEM_FP64     the above in FP64;
+//   HTR_CHEM_FP32_IR  the matrix equilibrated by powers of two and factorised
+//                     in FP32 (half the LDS of the factors); each stage solve
+//                     is refined against the FP64 matrix (r = b - A x in
+//                     FP64, x += LU32 \ r) until the correction is below 1e-15
+//                     of the solution; a solve whose refinement stalls makes
+//                     the step fall back to an FP64 LU of the same matrix,
+//                     factorised in place of the Jacobian (which is then
+//                     formed again if the step is retried).
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace htr {
 
-template<class MX, int PREC>
-struct Rosen {
-   static constexpr int NS = MX::NS, N = MX::NEQ;
-   const DMix &m;
-   const double *Ct;
-   double T;
+// LDS synchronisation of a lane group inside one wave: LDS operations of a
+// wave complete in order; the fences keep the compiler from moving them
+// across and wait for the outstanding ones.
+__device__ __forceinline__ void gsync()
+{
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
-   // ImplicitSolver::rhs
+// Offsets (in doubles) of one cell's LDS region.
+template<class MX, int PREC>
+struct ChemLayout {
+   static constexpr int NS = MX::NS, N = MX::NEQ;
+   enum { X, DX, XSAV, DXSAV, ERR, G1, G2, G3, G4, CT, FP, FM, RB, RR, VV, RS, CS, NVEC };
+   static constexpr int J = 0;
+   static constexpr int VEC = J + N*N;
+   static constexpr int SY = VEC + NVEC*N;             // mass fractions
+   static constexpr int SA = SY + NS;                  // species terms (energy; concentrations)
+   static constexpr int SB = SA + NS;                  // species terms (heat capacity; Gibbs energies)
+   static constexpr int SW = SB + NS;                  // production rates
+   static constexpr int IND = SW + NS;                 // pivot rows (int)
+   static constexpr int MISC = IND + (N + 1)/2;        // broadcast slots
+   static constexpr int SR = MISC + 4;                 // reaction rates, nReac entries
+   // the factors follow the nReac reaction rates: A = SR + nReac
+   static constexpr int a_doubles() { return (PREC == HTR_CHEM_FP64) ? N*N : (N*N + 1)/2; }
+   __host__ __device__ static int cell_doubles(int nreac) { return SR + nreac + a_doubles(); }
+};
+
+template<class MX, int PREC, int L>
+struct ChemGroup {
+   static constexpr int NS = MX::NS, N = MX::NEQ;
+   using Lay = ChemLayout<MX, PREC>;
+   using R = typename std::conditional<PREC == HTR_CHEM_FP64, double, float>::type;
+
+   const DMix &m;
+   double *s;          // this cell's LDS region
+   R      *A;          // the factors
+   int     lane;       // 0 .. L-1
+   double  T;          // temperature guess (uniform over the group)
+   bool    jac_valid;  // Jacobian present in LDS (FP32_IR: a fallback overwrites it)
+   bool    lu64;       // FP32_IR: this step's matrix is factorised in FP64 (in the Jacobian's place)
+   int     fallbacks;
+
+   __device__ double *v(int k) const { return s + Lay::VEC + k*N; }
+   __device__ double *Jm() const { return s + Lay::J; }
+   __device__ int *ind() const { return reinterpret_cast<int*>(s + Lay::IND); }
+
+   // in-order sum of n terms in LDS (every lane forms the same sum)
+   __device__ static double sum_in_order(const double *t, int n)
+   {
+      double a = 0.0;
+#pragma unroll 4
+      for (int i = 0; i < n; i++) a += t[i];
+      return a;
+   }
+
+   // group maximum (exact in any order)
+   __device__ static double gmax(double x)
+   {
+#pragma unroll
+      for (int o = L/2; o > 0; o >>= 1) x = dmax(x, __shfl_xor(x, o, L));
+      return x;
+   }
+
+   // MultiComponent.inl:187-210 over the group: the species terms of the
+   // energy and heat-capacity sums are formed by their lanes, the sums in
+   // order by every lane
+   __device__ double t_from_e(double e0, double Tg)
+   {
+      const double *Y = s + Lay::SY;
+      double *ta = s + Lay::SA, *tb = s + Lay::SB;
+      double dfdT = 1.0;
+      int j = 0;
+      double Tn = Tg;
+      while (j < 1000) {
+         const double Tr = Tn*m.TRef;
+         for (int i = lane; i < NS; i += L) ta[i] = Y[i]*(sp_enthalpy(m.sp[i], Tr) - RGAS*Tr/m.sp[i].W);
+         gsync();
+         const double f = e0 - sum_in_order(ta, NS)*m.ieRef;
+         if (fabs(f/dfdT) < 1e-8) break;
+         for (int i = lane; i < NS; i += L) tb[i] = Y[i]*(sp_cp(m.sp[i], Tn*m.TRef) - RGAS/m.sp[i].W)*m.iCpRef;
+         gsync();
+         dfdT = sum_in_order(tb, NS);
+         Tn += f/dfdT;
+         j++;
+         gsync();                      // the terms are rewritten by the next iteration
+      }
+      return Tn;
+   }
+
+   // ImplicitSolver::rhs (chem.hpp:53-93) of x (LDS) into r (LDS)
    __device__ void rhs(double *r, const double *x)
    {
-      double rhoYi[NS], Yi[NS], w[NS];
-#pragma unroll
-      for (int i = 0; i < NS; i++) rhoYi[i] = x[i];
-      double rho = 0.0;
-#pragma unroll
-      for (int i = 0; i < NS; i++) rho += rhoYi[i];
+      double *Y = s + Lay::SY, *ta = s + Lay::SA, *tb = s + Lay::SB, *w = s + Lay::SW, *Rr = s + Lay::SR;
+      const double *ct = v(Lay::CT);
+      const double rho = sum_in_order(x, NS);
       const double irho = 1.0/rho;
-#pragma unroll
-      for (int i = 0; i < NS; i++) { Yi[i] = irho*rhoYi[i]; Yi[i] = dmax(Yi[i], 1e-60); Yi[i] = dmin(Yi[i], 1.0); }
-      const double MixW = MX::MolarWeightFromY(m, Yi);
+      for (int i = lane; i < NS; i += L) {
+         double y = irho*x[i];
+         y = dmax(y, 1e-60);
+         y = dmin(y, 1.0);
+         Y[i] = y;
+         ta[i] = y/m.sp[i].W;
+      }
+      gsync();
+      const double MixW = 1.0/sum_in_order(ta, NS);
       const double rhoInv = 1.0/rho;
-      double v[3];
+      double vel[3];
 #pragma unroll
-      for (int i = 0; i < 3; i++) v[i] = x[NS+i]*rhoInv;
-      double k2 = 0.0; k2 += v[0]*v[0]; k2 += v[1]*v[1]; k2 += v[2]*v[2];
+      for (int i = 0; i < 3; i++) vel[i] = x[NS+i]*rhoInv;
+      double k2 = 0.0; k2 += vel[0]*vel[0]; k2 += vel[1]*vel[1]; k2 += vel[2]*vel[2];
       const double e = x[NS+3]*rhoInv - 0.5*k2;
-      T = MX::TFromInternalEnergy(m, e, T, Yi);
+      gsync();                         // ta is reused by the Newton
+      T = t_from_e(e, T);
       const double P = MX::PFromRhoAndT(m, rho, MixW, T);
-      MX::ProductionRates(m, w, rho, P, T, Yi);
-#pragma unroll
-      for (int i = 0; i < NS; i++) r[i] = w[i] + Ct[i];
-#pragma unroll
-      for (int i = NS; i < N; i++) r[i] = Ct[i];
+      // GetProductionRates (MultiComponent.inl:371-417, Reaction.inl:34-228)
+      const double Tr = T*m.TRef, Pr = P*m.PRef, rr = rho*m.rhoRef;
+      for (int i = lane; i < NS; i += L) {
+         ta[i] = Y[i]*rr/m.sp[i].W;                     // concentration
+         tb[i] = sp_free_enthalpy(m.sp[i], Tr);         // Gibbs energy
+      }
+      gsync();
+      const double logT = log(Tr);
+      const double logRP = log(RGAS/Pr);
+      for (int k = lane; k < m.nReac; k += L) {
+         const DReac &rc = m.rc[k];
+         const double Kf = MX::RateCoeff(rc.A, rc.n, rc.EovR, Tr);
+         double a = 1.0;
+         for (int i = 0; i < rc.Ne; i++) {
+            const int id = rc.ed_ind[i];
+            if (rc.ed_nu[i] == 1) a *= ta[id];
+            else                  a *= pow(ta[id], (double)rc.ed_nu[i]);
+         }
+         double Kb = 0.0, b = 1.0;
+         if (rc.has_backward) {
+            double sumNu = 0.0, sumNuG = 0.0;
+            for (int i = 0; i < rc.Ne; i++) { sumNu -= rc.ed_nu[i]; sumNuG -= rc.ed_nu[i]*tb[rc.ed_ind[i]]; }
+            for (int i = 0; i < rc.Np; i++) { sumNu += rc.pd_nu[i]; sumNuG += rc.pd_nu[i]*tb[rc.pd_ind[i]]; }
+            const double lnKc = -sumNuG - sumNu*(logT + logRP);
+            Kb = Kf*exp(-lnKc);
+            for (int i = 0; i < rc.Np; i++) {
+               const int id = rc.pd_ind[i];
+               if (rc.pd_nu[i] == 1) b *= ta[id];
+               else                  b *= pow(ta[id], (double)rc.pd_nu[i]);
+            }
+         }
+         double q = Kf*a - Kb*b;
+         if (rc.kind == REAC_THIRDB) {
+            double c = 1.0;
+            if (rc.Ntb > 0) {
+               c = Pr/(RGAS*Tr);
+               for (int i = 0; i < rc.Ntb; i++) c += ta[rc.tb_ind[i]]*(rc.tb_eff[i] - 1.0);
+            }
+            q = c*(Kf*a - Kb*b);
+         }
+         Rr[k] = q;
+      }
+      gsync();
+      for (int i = lane; i < NS; i += L) {
+         double wi = 0.0;
+         for (int e2 = m.spr_ptr[i]; e2 < m.spr_ptr[i+1]; e2++) wi += m.spr_nu[e2]*Rr[m.spr_reac[e2]];
+         w[i] = wi*(m.sp[i].W*m.iwiRef);
+      }
+      gsync();
+      for (int i = lane; i < N; i += L) r[i] = (i < NS) ? w[i] + ct[i] : ct[i];
+      gsync();
    }
 
-   // ---- LU, column-major A(i,j) = A[i + j*N] ----------------------------------
-   template<typename R>
-   __device__ static void ludcmp(R *A, int *ind)
-   {
-      int imax = 0;
-      R vv[N];
-      for (int i = 0; i < N; i++) {
-         R big = 0;
-         for (int j = 0; j < N; j++) { const R a = fabs(A[i + j*N]); big = (big < a) ? a : big; }
-         vv[i] = (R)1/big;
-      }
-      for (int j = 0; j < N; j++) {
-         for (int i = 0; i < j; i++) {
-            R sum = A[i + j*N];
-            for (int k = 0; k < i; k++) sum -= A[i + k*N]*A[k + j*N];
-            A[i + j*N] = sum;
-         }
-         R big = 0;
-         for (int i = j; i < N; i++) {
-            R sum = A[i + j*N];
-            for (int k = 0; k < j; k++) sum -= A[i + k*N]*A[k + j*N];
-            A[i + j*N] = sum;
-            const R dum = vv[i]*fabs(sum);
-            if (dum >= big) { big = dum; imax = i; }
-         }
-         if (j != imax) {
-            for (int k = 0; k < N; k++) { const R d = A[imax + k*N]; A[imax + k*N] = A[j + k*N]; A[j + k*N] = d; }
-            vv[imax] = vv[j];
-         }
-         ind[j] = imax;
-         if (A[j + j*N] == (R)0) A[j + j*N] = (R)1.0e-20;
-         if (j != N-1) {
-            const R dum = (R)1/(A[j + j*N]);
-            for (int i = j+1; i < N; i++) A[i + j*N] *= dum;
-         }
-      }
-   }
-
-   template<typename R, typename V>
-   __device__ static void lubksb(const R *A, const int *ind, V *b)
-   {
-      int ii = 0;
-      for (int i = 0; i < N; i++) {
-         const int ip = ind[i];
-         V sum = b[ip];
-         b[ip] = b[i];
-         if (ii != 0)
-            for (int j = ii-1; j < i; j++) sum -= (V)A[i + j*N]*b[j];
-         else if (sum != (V)0)
-            ii = i+1;
-         b[i] = sum;
-      }
-      for (int i = N-1; i > -1; i--) {
-         V sum = b[i];
-         for (int j = i+1; j < N; j++) sum -= (V)A[i + j*N]*b[j];
-         b[i] = sum/(V)A[i + i*N];
-      }
-   }
-
-   double Jac[N*N];
-   double A64[N*N];                                   // FP64: the LU itself; IR: the unfactorised matrix
-   float  A32[PREC == HTR_CHEM_FP64 ? 1 : N*N];       // IR: the scaled FP32 LU
-   double L64[PREC == HTR_CHEM_FP64 ? 1 : N*N];       // IR: FP64 LU made only when refinement stalls
-   double rs[PREC == HTR_CHEM_FP64 ? 1 : N], cs[PREC == HTR_CHEM_FP64 ? 1 : N];
-   int    ind[N], ind64[PREC == HTR_CHEM_FP64 ? 1 : N];
-   bool   lu64_ready;
-   int    fallbacks = 0;
-   double err[N], dx[N], xsav[N], dxsav[N];
-
+   // Rosenbrock::GetJacobian (math_utils.hpp:249-269), column by column
    __device__ void jacobian(double *x)
    {
-      double tmp[N], fp[N], fm[N];
-#pragma unroll
-      for (int j = 0; j < N; j++) tmp[j] = x[j];
+      double *fp = v(Lay::FP), *fm = v(Lay::FM), *J = Jm();
       for (int j = 0; j < N; j++) {
-         const double h = x[j]*1.0e-6 + 1.0e-14;
-         x[j] = tmp[j] + h;
-         const double hp = x[j] - tmp[j];
+         const double tmp = x[j];
+         const double h = tmp*1.0e-6 + 1.0e-14;
+         const double xp = tmp + h;
+         const double hp = xp - tmp;
+         gsync();
+         if (lane == 0) x[j] = xp;
+         gsync();
          rhs(fp, x);
-         x[j] = tmp[j] - h;
-         const double hm = tmp[j] - x[j];
+         const double xm = tmp - h;
+         const double hm = tmp - xm;
+         if (lane == 0) x[j] = xm;
+         gsync();
          rhs(fm, x);
-         x[j] = tmp[j];
-         for (int i = 0; i < N; i++) Jac[i + j*N] = (fp[i] - fm[i])/(hp + hm);
+         if (lane == 0) x[j] = tmp;
+         for (int i = lane; i < N; i += L) J[i + j*N] = (fp[i] - fm[i])/(hp + hm);
+         gsync();
       }
+      jac_valid = true;
    }
 
    // power of two nearest to 1/a (exact scaling)
    __device__ static double pow2_inv(double a)
    {
       if (!(a > 0.0) || !(a < 1.0e308)) return 1.0;
-      int e;
-      frexp(a, &e);
-      return ldexp(1.0, -e);
+      // a = m 2^e with m in [0.5, 1): e from the biased exponent (normal a)
+      const int be = (int)((__double_as_longlong(a) >> 52) & 0x7ff);
+      if (be == 0) return 1.0;
+      return ldexp(1.0, -(be - 1022));
    }
 
-   __device__ void factor(double dt)
+   // LUdec::ludcmp (math_utils.hpp:87-155) of the N x N column-major matrix M
+   template<typename T_>
+   __device__ void ludcmp(T_ *M)
    {
-      for (int j = 0; j < N; j++)
-         for (int i = 0; i < N; i++) A64[i + j*N] = -Jac[i + j*N];
-      for (int i = 0; i < N; i++) A64[i + i*N] += 1.0/((1.0/2.0)*dt);
-      if (PREC == HTR_CHEM_FP64) {
-         ludcmp<double>(A64, ind);
-      } else {
-         // equilibrate by powers of two (rows, then columns) before rounding to
-         // FP32: the FP32 LU then sees entries of comparable magnitude
-         for (int i = 0; i < N; i++) {
-            double big = 0.0;
-            for (int j = 0; j < N; j++) big = dmax(big, fabs(A64[i + j*N]));
-            rs[i] = pow2_inv(big);
+      double *vv = v(Lay::VV);
+      int *piv = ind();
+      for (int i = lane; i < N; i += L) {
+         T_ big = 0;
+         for (int j = 0; j < N; j++) { const T_ a = fabs(M[i + j*N]); big = (big < a) ? a : big; }
+         vv[i] = (double)((T_)1/big);
+      }
+      gsync();
+      for (int j = 0; j < N; j++) {
+         // pivot: the last row i >= j with the largest vv[i] |M(i, j)|
+         double best = -1.0;
+         int bi = -1;
+         for (int i = j + lane; i < N; i += L) {
+            const T_ dum = (T_)vv[i]*fabs(M[i + j*N]);
+            if ((double)dum >= best) { best = (double)dum; bi = i; }
          }
-         for (int j = 0; j < N; j++) {
-            double big = 0.0;
-            for (int i = 0; i < N; i++) big = dmax(big, fabs(rs[i]*A64[i + j*N]));
-            cs[j] = pow2_inv(big);
+#pragma unroll
+         for (int o = L/2; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, L);
+            const int oi = __shfl_xor(bi, o, L);
+            if (ob > best || (ob == best && oi > bi)) { best = ob; bi = oi; }
          }
-         for (int j = 0; j < N; j++)
-            for (int i = 0; i < N; i++) A32[i + j*N] = (float)(rs[i]*A64[i + j*N]*cs[j]);
-         ludcmp<float>(A32, ind);
-         lu64_ready = false;
+         const int imax = (bi < 0) ? j : bi;
+         if (imax != j) {
+            for (int k = lane; k < N; k += L) {
+               const T_ d = M[imax + k*N];
+               M[imax + k*N] = M[j + k*N];
+               M[j + k*N] = d;
+            }
+            gsync();
+            if (lane == 0) vv[imax] = vv[j];
+         }
+         if (lane == 0) {
+            piv[j] = imax;
+            if (M[j + j*N] == (T_)0) M[j + j*N] = (T_)1.0e-20;
+         }
+         gsync();
+         if (j != N-1) {
+            const T_ dum = (T_)1/(M[j + j*N]);
+            for (int i = j + 1 + lane; i < N; i += L) M[i + j*N] *= dum;
+            gsync();
+            // trailing update: M(i, k) -= M(i, j) M(j, k), i, k > j
+            const int w = N - j - 1;
+            for (int e = lane; e < w*w; e += L) {
+               const int i = j + 1 + e % w, k = j + 1 + e/w;
+               M[i + k*N] -= M[i + j*N]*M[j + k*N];
+            }
+            gsync();
+         }
       }
    }
 
-   // x = A^-1 r through the scaled FP32 LU
-   __device__ void solve32(const double *r, double *x)
+   // LUdec::lubksb (math_utils.hpp:157-188): permutation, forward
+   // substitution column by column (each entry receives its products in the
+   // reference's order), back substitution row by row by one lane in the
+   // reference's order
+   template<typename T_, typename V>
+   __device__ void lubksb(const T_ *M, V *b)
    {
-      float c[N];
-#pragma unroll
-      for (int i = 0; i < N; i++) c[i] = (float)(rs[i]*r[i]);
-      lubksb<float, float>(A32, ind, c);
-#pragma unroll
-      for (int i = 0; i < N; i++) x[i] = cs[i]*(double)c[i];
+      const int *piv = ind();
+      if (lane == 0)
+         for (int i = 0; i < N; i++) { const int ip = piv[i]; const V t = b[ip]; b[ip] = b[i]; b[i] = t; }
+      gsync();
+      for (int i = 0; i < N - 1; i++) {
+         const V bi = b[i];
+         for (int k = i + 1 + lane; k < N; k += L) b[k] -= (V)M[k + i*N]*bi;
+         gsync();
+      }
+      if (lane == 0)
+         for (int i = N - 1; i > -1; i--) {
+            V sum = b[i];
+            for (int j = i + 1; j < N; j++) sum -= (V)M[i + j*N]*b[j];
+            b[i] = sum/(V)M[i + i*N];
+         }
+      gsync();
    }
 
-   __device__ void solve(double *b)
+   // A = I/(gam dt) - J, factorised (Rosenbrock::step, math_utils.hpp:282-291)
+   __device__ void factor(double dt, double *x)
    {
-      if (PREC == HTR_CHEM_FP64) { lubksb<double, double>(A64, ind, b); return; }
-      // FP32 solve + FP64 residual refinement, until the correction is below
-      // 1e-15 of the solution; a stalled refinement (the correction shrinks by
-      // less than half in a sweep) falls back to an FP64 LU of the same matrix
-      double rhs0[N], x[N], r[N], c[N];
-#pragma unroll
-      for (int i = 0; i < N; i++) rhs0[i] = b[i];
-      solve32(rhs0, x);
+      const double d = 1.0/((1.0/2.0)*dt);
+      if (!jac_valid) jacobian(x);     // after an FP64 fallback: formed again at the step's start point
+      if (PREC == HTR_CHEM_FP64) {
+         double *M = reinterpret_cast<double*>(A);
+         const double *J = Jm();
+         for (int e = lane; e < N*N; e += L) M[e] = -J[e];
+         gsync();
+         for (int i = lane; i < N; i += L) M[i + i*N] += d;
+         gsync();
+         ludcmp<double>(M);
+      } else {
+         // equilibrate by powers of two (rows, then columns) before rounding
+         // to FP32
+         const double *J = Jm();
+         double *rs = v(Lay::RS), *cs = v(Lay::CS);
+         for (int i = lane; i < N; i += L) {
+            double big = 0.0;
+            for (int j = 0; j < N; j++) big = dmax(big, fabs(-J[i + j*N] + ((i == j) ? d : 0.0)));
+            rs[i] = pow2_inv(big);
+         }
+         gsync();
+         for (int j = lane; j < N; j += L) {
+            double big = 0.0;
+            for (int i = 0; i < N; i++) big = dmax(big, fabs(rs[i]*(-J[i + j*N] + ((i == j) ? d : 0.0))));
+            cs[j] = pow2_inv(big);
+         }
+         gsync();
+         float *M = reinterpret_cast<float*>(A);
+         for (int e = lane; e < N*N; e += L) {
+            const int i = e % N, j = e/N;
+            M[e] = (float)(rs[i]*(-J[e] + ((i == j) ? d : 0.0))*cs[j]);
+         }
+         gsync();
+         ludcmp<float>(M);
+         lu64 = false;
+      }
+   }
+
+   // FP32_IR fallback: the FP64 LU of the same matrix, in the Jacobian's place
+   __device__ void factor64_in_place(double dt)
+   {
+      const double d = 1.0/((1.0/2.0)*dt);
+      double *M = Jm();
+      for (int e = lane; e < N*N; e += L) M[e] = -M[e];
+      gsync();
+      for (int i = lane; i < N; i += L) M[i + i*N] += d;
+      gsync();
+      ludcmp<double>(M);
+      jac_valid = false;
+      lu64 = true;
+      fallbacks++;
+   }
+
+   // b <- A^-1 b
+   __device__ void solve(double *b, double dt)
+   {
+      if (PREC == HTR_CHEM_FP64) { lubksb<double, double>(reinterpret_cast<double*>(A), b); return; }
+      if (lu64) { lubksb<double, double>(Jm(), b); return; }
+      const float *M = reinterpret_cast<const float*>(A);
+      const double *J = Jm();
+      const double *rs = v(Lay::RS), *cs = v(Lay::CS);
+      double *rb = v(Lay::RB), *rr = v(Lay::RR);
+      const double d = 1.0/((1.0/2.0)*dt);
+      // rb = b; x = 0; refine from the zero solution: the first sweep is the
+      // plain FP32 solve
+      for (int i = lane; i < N; i += L) { rb[i] = b[i]; b[i] = 0.0; }
+      gsync();
       bool done = false;
       double prev = 0.0;
-      for (int it = 0; it < 8 && !done; it++) {
-         double nx = 0.0;
-         for (int i = 0; i < N; i++) {
-            double s = rhs0[i];
-            for (int j = 0; j < N; j++) s -= A64[i + j*N]*x[j];
-            r[i] = s;
-            nx = dmax(nx, fabs(x[i]));
+      for (int it = 0; it < 9 && !done; it++) {
+         // residual r = rb - A x in FP64, scaled by rs for the FP32 solve
+         for (int i = lane; i < N; i += L) {
+            double sacc = rb[i];
+            for (int j = 0; j < N; j++) sacc -= (((i == j) ? d : 0.0) - J[i + j*N])*b[j];
+            rr[i] = rs[i]*sacc;
          }
-         solve32(r, c);
-         double nc = 0.0;
-#pragma unroll
-         for (int i = 0; i < N; i++) { x[i] += c[i]; nc = dmax(nc, fabs(c[i])); }
+         gsync();
+         lubksb<float, double>(M, rr);
+         double nx = 0.0, nc = 0.0;
+         for (int i = lane; i < N; i += L) {
+            const double ci = cs[i]*rr[i];
+            b[i] += ci;
+            nc = dmax(nc, fabs(ci));
+            nx = dmax(nx, fabs(b[i]));
+         }
+         gsync();
+         nc = gmax(nc);
+         nx = gmax(nx);
+         if (it == 0) { prev = nc; continue; }
          if (!(nc > 1e-15*nx)) done = true;
-         else if (it > 0 && !(nc < 0.5*prev)) break;
+         else if (!(nc < 0.5*prev)) break;
          prev = nc;
       }
       if (!done) {
-         if (!lu64_ready) {
-            for (int k = 0; k < N*N; k++) L64[k] = A64[k];
-            ludcmp<double>(L64, ind64);
-            lu64_ready = true;
-         }
-#pragma unroll
-         for (int i = 0; i < N; i++) x[i] = rhs0[i];
-         lubksb<double, double>(L64, ind64, x);
-         fallbacks++;
+         // refinement stalled: FP64 LU of the same matrix for this step
+         factor64_in_place(dt);
+         for (int i = lane; i < N; i += L) b[i] = rb[i];
+         gsync();
+         lubksb<double, double>(Jm(), b);
       }
-#pragma unroll
-      for (int i = 0; i < N; i++) b[i] = x[i];
    }
 
+   // Rosenbrock::step (math_utils.hpp:282-330): Kaps-Rentrop, gamma = 1/2
    __device__ double step(double *x, double dt, double t0)
    {
-      double g1[N], g2[N], g3[N], g4[N];
-      factor(dt);
-#pragma unroll
-      for (int i = 0; i < N; i++) g1[i] = dxsav[i] + dt*(1.0/2.0)*dx[i];
-      solve(g1);
-#pragma unroll
-      for (int i = 0; i < N; i++) x[i] = xsav[i] + 2.0*g1[i];
+      double *dx = v(Lay::DX), *xsav = v(Lay::XSAV), *dxsav = v(Lay::DXSAV), *err = v(Lay::ERR);
+      double *g1 = v(Lay::G1), *g2 = v(Lay::G2), *g3 = v(Lay::G3), *g4 = v(Lay::G4);
+      factor(dt, xsav);
+      for (int i = lane; i < N; i += L) g1[i] = dxsav[i] + dt*(1.0/2.0)*dx[i];
+      gsync();
+      solve(g1, dt);
+      for (int i = lane; i < N; i += L) x[i] = xsav[i] + 2.0*g1[i];
+      gsync();
       rhs(dx, x);
-#pragma unroll
-      for (int i = 0; i < N; i++) g2[i] = dx[i] + dt*(-3.0/2.0)*dx[i] + (-8.0)*g1[i]/dt;
-      solve(g2);
-#pragma unroll
-      for (int i = 0; i < N; i++) x[i] = xsav[i] + (48.0/25.0)*g1[i] + (6.0/25.0)*g2[i];
+      for (int i = lane; i < N; i += L) g2[i] = dx[i] + dt*(-3.0/2.0)*dx[i] + (-8.0)*g1[i]/dt;
+      gsync();
+      solve(g2, dt);
+      for (int i = lane; i < N; i += L) x[i] = xsav[i] + (48.0/25.0)*g1[i] + (6.0/25.0)*g2[i];
+      gsync();
       rhs(dx, x);
-#pragma unroll
-      for (int i = 0; i < N; i++) g3[i] = dx[i] + dt*(121.0/50.0)*dx[i] + ((372.0/25.0)*g1[i] + (12.0/5.0)*g2[i])/dt;
-      solve(g3);
-#pragma unroll
-      for (int i = 0; i < N; i++)
+      for (int i = lane; i < N; i += L)
+         g3[i] = dx[i] + dt*(121.0/50.0)*dx[i] + ((372.0/25.0)*g1[i] + (12.0/5.0)*g2[i])/dt;
+      gsync();
+      solve(g3, dt);
+      for (int i = lane; i < N; i += L)
          g4[i] = dx[i] + dt*(29.0/250.0)*dx[i] + ((-112.0/125.0)*g1[i] + (-54.0/125.0)*g2[i] + (-2.0/5.0)*g3[i])/dt;
-      solve(g4);
-#pragma unroll
-      for (int i = 0; i < N; i++)
+      gsync();
+      solve(g4, dt);
+      for (int i = lane; i < N; i += L) {
          x[i] = xsav[i] + (19.0/9.0)*g1[i] + (1.0/2.0)*g2[i] + (25.0/108.0)*g3[i] + (125.0/108.0)*g4[i];
-#pragma unroll
-      for (int i = 0; i < N; i++)
          err[i] = (17.0/54.0)*g1[i] + (7.0/36.0)*g2[i] + 0.0*g3[i] + (125.0/108.0)*g4[i];
+      }
+      gsync();
       return t0 + dt;
    }
 
+   // Rosenbrock::itercheck (math_utils.hpp:337-360)
    __device__ bool itercheck(double &dt, double &dtNext, bool finish)
    {
+      const double *err = v(Lay::ERR);
       double errmax = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; i++) errmax = dmax(errmax, fabs(err[i]));
+      for (int i = lane; i < N; i += L) errmax = dmax(errmax, fabs(err[i]));
+      errmax = gmax(errmax);
       errmax *= (1.0/1e-10);
       if ((errmax < 1.0) || finish) {
          if (errmax > 0.1296) dtNext = 0.9*dt*pow(errmax, -0.25);
@@ -288,9 +483,11 @@ struct Rosen {
       return false;
    }
 
-   // returns accepted steps, or -1 if a reference cap was hit
-   __device__ int solve_cell(double *x, double dtTry, double DelT)
+   // Rosenbrock::solve (math_utils.hpp:195-247, 362-386); returns accepted
+   // steps, or -1 if a reference cap was hit
+   __device__ int solve_cell(double dtTry, double DelT)
    {
+      double *x = v(Lay::X), *dx = v(Lay::DX), *xsav = v(Lay::XSAV), *dxsav = v(Lay::DXSAV);
       bool finish = false;
       double time = 0.0;
       double dt = dtTry;
@@ -299,8 +496,8 @@ struct Rosen {
          double dtNext = 0.0;
          jacobian(x);
          rhs(dx, x);
-#pragma unroll
-         for (int i = 0; i < N; i++) { xsav[i] = x[i]; dxsav[i] = dx[i]; }
+         for (int i = lane; i < N; i += L) { xsav[i] = x[i]; dxsav[i] = dx[i]; }
+         gsync();
          int jtry;
          for (jtry = 0; jtry < 100; jtry++) {
             time = step(x, dt, t0);
@@ -316,63 +513,111 @@ struct Rosen {
    }
 };
 
-template<class MX, int PREC>
-__global__ void __launch_bounds__(256)
-chem_update_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double dt,
-                   htr_view nsteps, int *__restrict__ fails, int64_t n)
+// One lane group of L lanes per cell; a block is one wave (64/L groups); the
+// groups take cells from the queue counter until it passes n.
+template<class MX, int PREC, int L>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+chem_group_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double dt,
+                  htr_view nsteps, int *__restrict__ fails, int *__restrict__ queue, int cell_doubles,
+                  int64_t n)
 {
+   extern __shared__ double lds[];
+   using Lay = ChemLayout<MX, PREC>;
    constexpr int N = MX::NEQ;
-   for (int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x*blockDim.x) {
-      const P3 p = exec_point(g, i);
-      double x[N], ct[N];
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-         ct[k] = ld<double>(f.Conserved_t, p, k);
-         at<double>(f.Conserved_t_old, p, k) = ct[k];
+   const int grp = threadIdx.x/L;
+   const int lane = threadIdx.x % L;
+   double *s = lds + grp*cell_doubles;
+   ChemGroup<MX, PREC, L> G{*mix, s, reinterpret_cast<typename ChemGroup<MX, PREC, L>::R*>(s + Lay::SR + mix->nReac),
+                            lane, 0.0, false, false, 0};
+   int *slot = reinterpret_cast<int*>(s + Lay::MISC);
+   for (;;) {
+      if (lane == 0) slot[0] = atomicAdd(queue, 1);
+      gsync();
+      const int64_t c = slot[0];
+      gsync();
+      if (c >= n) break;
+      const P3 p = exec_point(g, c);
+      double *x = G.v(Lay::X), *ct = G.v(Lay::CT);
+      for (int k = lane; k < N; k += L) {
+         const double t = ld<double>(f.Conserved_t, p, k);
+         at<double>(f.Conserved_t_old, p, k) = t;
+         ct[k] = t;
          x[k] = ld<double>(f.Conserved, p, k);
       }
-      Rosen<MX, PREC> R{*mix, ct, ld<double>(f.temperature, p)};
-      const int st = R.solve_cell(x, dt, dt);
-#pragma unroll
-      for (int k = 0; k < N; k++) at<double>(f.Conserved, p, k) = x[k];
-      at<double>(f.temperature, p) = R.T;
-      if (nsteps.ptr) at<int32_t>(nsteps, p) = st;
-      if (st < 0) atomicAdd(fails, 1);
+      G.T = ld<double>(f.temperature, p);
+      G.jac_valid = false;
+      G.lu64 = false;
+      gsync();
+      const int st = G.solve_cell(dt, dt);
+      for (int k = lane; k < N; k += L) at<double>(f.Conserved, p, k) = x[k];
+      if (lane == 0) {
+         at<double>(f.temperature, p) = G.T;
+         if (nsteps.ptr) at<int32_t>(nsteps, p) = st;
+         if (st < 0) atomicAdd(fails, 1);
+      }
+      gsync();
    }
 }
 
-// Launch shape of the persistent cell loop: threads per block and resident
-// blocks per CU (256 CUs); set by htr_chem_launch_config, chosen by the
-// autotuner in htr_amd.chem_autotune.
-static int g_chem_block = 64;
-static int g_chem_blocks_per_cu = 16;
+// Launch shape: lanes per cell (cells per wave = 64 / lanes) and resident
+// blocks (waves) per CU; set by htr_chem_launch_config, chosen by the
+// autotuner in htr_amd.chem_autotune.  0 = the default of the mixture.
+static int g_chem_lanes = 0;
+static int g_chem_blocks_per_cu = 0;
 
-int chem_set_launch(int block, int blocks_per_cu)
+int chem_set_launch(int lanes, int blocks_per_cu)
 {
-   if (block != 64 && block != 128 && block != 256) return set_error(HTR_E_ARG, "chem block size %d", block);
-   if (blocks_per_cu < 1 || blocks_per_cu > 64) return set_error(HTR_E_ARG, "chem blocks per CU %d", blocks_per_cu);
-   g_chem_block = block;
+   if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+      return set_error(HTR_E_ARG, "chem lanes per cell %d", lanes);
+   if (blocks_per_cu < 0 || blocks_per_cu > 32) return set_error(HTR_E_ARG, "chem blocks per CU %d", blocks_per_cu);
+   g_chem_lanes = lanes;
    g_chem_blocks_per_cu = blocks_per_cu;
    return HTR_OK;
 }
 
+template<class MX, int PREC, int L>
+static int chem_launch_l(const htr_fields &f, const DMix *mix, const Geo &g, double dt, const htr_view &nsteps,
+                         Workspace &w, int nreac, int64_t n, hipStream_t s)
+{
+   using Lay = ChemLayout<MX, PREC>;
+   const int cd = Lay::cell_doubles(nreac);
+   const size_t shm = (size_t)(64/L)*cd*sizeof(double);
+   if (shm > 160*1024) return set_error(HTR_E_UNSUPPORTED, "chem: %zu B of LDS per wave", shm);
+   int per_cu = g_chem_blocks_per_cu;
+   if (per_cu == 0) per_cu = std::max(1, std::min(16, (int)((160*1024)/shm)));
+   const int64_t groups = (int64_t)256*per_cu*(64/L);
+   const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 64/L - 1)/(64/L), groups/(64/L)));
+   HTR_LAUNCH(PREC == HTR_CHEM_FP64 ? "chem_update_fp64" : "chem_update_fp32ir", (chem_group_kernel<MX, PREC, L>),
+              dim3(blocks), dim3(64), shm, s, f, mix, g, dt, nsteps, w.counter, w.counter + 3, cd, n);
+   return check_launch("chem_group_kernel");
+}
+
+template<class MX, int PREC>
+static int chem_launch_p(const htr_fields &f, const DMix *mix, const Geo &g, double dt, const htr_view &nsteps,
+                         Workspace &w, int nreac, int64_t n, hipStream_t s)
+{
+   int lanes = g_chem_lanes;
+   if (lanes == 0) lanes = (MX::NS >= 16) ? 64 : 16;
+   switch (lanes) {
+      case 8:  return chem_launch_l<MX, PREC, 8>(f, mix, g, dt, nsteps, w, nreac, n, s);
+      case 16: return chem_launch_l<MX, PREC, 16>(f, mix, g, dt, nsteps, w, nreac, n, s);
+      case 32: return chem_launch_l<MX, PREC, 32>(f, mix, g, dt, nsteps, w, nreac, n, s);
+      default: return chem_launch_l<MX, PREC, 64>(f, mix, g, dt, nsteps, w, nreac, n, s);
+   }
+}
+
 template<class MX>
 int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, double dt, int precision,
-                       const htr_view &nsteps, hipStream_t s)
+                       const htr_view &nsteps, int nreac, hipStream_t s)
 {
    const int64_t n = exec_cells(g);
    if (n == 0) return HTR_OK;
    Workspace *wp = workspace(s);
    if (!wp) return HTR_E_DEVICE;
    Workspace &w = *wp;
-   if (int rc0 = zero_counters(w.counter, s)) return rc0;
-   const int bs = g_chem_block;
-   const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, bs), 256*g_chem_blocks_per_cu);
-   if (precision == HTR_CHEM_FP64)
-      HTR_LAUNCH("chem_update_fp64", (chem_update_kernel<MX, HTR_CHEM_FP64>), dim3(blocks), dim3(bs), 0, s, f, mix, g, dt, nsteps, w.counter, n);
-   else
-      HTR_LAUNCH("chem_update_fp32ir", (chem_update_kernel<MX, HTR_CHEM_FP32_IR>), dim3(blocks), dim3(bs), 0, s, f, mix, g, dt, nsteps, w.counter, n);
-   int rc = check_launch("chem_update_kernel");
+   if (int rc0 = zero_counters(w.counter, s)) return rc0;      // failures and the cell queue
+   int rc = (precision == HTR_CHEM_FP64) ? chem_launch_p<MX, HTR_CHEM_FP64>(f, mix, g, dt, nsteps, w, nreac, n, s)
+                                         : chem_launch_p<MX, HTR_CHEM_FP32_IR>(f, mix, g, dt, nsteps, w, nreac, n, s);
    if (rc) return rc;
    int fails = 0;
    if (hipMemcpyAsync(&fails, w.counter, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -382,7 +627,9 @@ int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, doubl
    return HTR_OK;
 }
 
-template int chem_update_launch<MixAir>(const htr_fields &, const DMix *, const Geo &, double, int, const htr_view &, hipStream_t);
-template int chem_update_launch<Mix30>(const htr_fields &, const DMix *, const Geo &, double, int, const htr_view &, hipStream_t);
+template int chem_update_launch<MixAir>(const htr_fields &, const DMix *, const Geo &, double, int, const htr_view &,
+                                        int, hipStream_t);
+template int chem_update_launch<Mix30>(const htr_fields &, const DMix *, const Geo &, double, int, const htr_view &,
+                                       int, hipStream_t);
 
 } // namespace htr

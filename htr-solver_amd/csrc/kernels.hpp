@@ -56,7 +56,7 @@ int ducros_launch(const htr_fields &f, const Geo &g, double vorticityScale, hipS
 template<class MX> int chem_source_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s,
                                           bool zero = false);
 template<class MX> int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, double dt,
-                                          int precision, const htr_view &nsteps, hipStream_t s);
+                                          int precision, const htr_view &nsteps, int nreac, hipStream_t s);
 template<class MX> int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g,
                                               double *result, hipStream_t s);
 int chem_set_launch(int block, int blocks_per_cu);

@@ -51,6 +51,8 @@ struct DReac {
    float  tb_eff[MAX_TB];
 };
 
+constexpr int MAX_SPR  = MAX_REAC*(MAX_ED + MAX_PD);
+
 struct DMix {
    int    constprop, nSpec, nReac, viscModel;
    double R, gamma, Prandtl;
@@ -60,6 +62,14 @@ struct DMix {
    double XiRef[MAX_SP];
    DSpec  sp[MAX_SP];
    DReac  rc[MAX_REAC];
+   // Species-major image of the stoichiometry, for the lane-per-species
+   // accumulation of the production rates: the entries of species s are
+   // [spr_ptr[s], spr_ptr[s+1]), in the order the reaction loop of
+   // GetProductionRates adds them (reaction order; educt entries, then
+   // product entries), with the signed coefficient (-nu educt, +nu product).
+   int     spr_ptr[MAX_SP + 1];
+   int16_t spr_reac[MAX_SPR];
+   float   spr_nu[MAX_SPR];
 };
 
 #define HD __host__ __device__ inline

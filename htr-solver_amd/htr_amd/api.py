@@ -317,17 +317,19 @@ def profile_report():
     return out
 
 
-def chem_launch_config(block, blocks_per_cu):
-    """Launch shape of the persistent implicit-chemistry cell loop."""
-    check(lib().htr_chem_launch_config(int(block), int(blocks_per_cu)))
+def chem_launch_config(lanes_per_cell, blocks_per_cu=0):
+    """Launch shape of the implicit chemistry: lanes of a wave per cell (8, 16,
+    32, 64; 0 = default) and resident one-wave blocks per CU (0 = as many as
+    the LDS holds)."""
+    check(lib().htr_chem_launch_config(int(lanes_per_cell), int(blocks_per_cu)))
 
 
-CHEM_CONFIGS = [(64, 4), (64, 8), (64, 16), (128, 4), (128, 8), (256, 2), (256, 4)]
+CHEM_CONFIGS = [(8, 0), (16, 0), (32, 0), (64, 0), (64, 2), (64, 4)]
 
 
 def chem_autotune(fs, mix, deltaTime, precision=HTR_CHEM_FP64, grid=None, configs=None, repeats=1):
-    """Times update_chemistry over `grid` for each launch shape (block size,
-    resident blocks per CU) on a copy of the cell states (restored after every
+    """Times update_chemistry over `grid` for each launch shape (lanes per
+    cell, resident blocks per CU) on a copy of the cell states (restored after every
     trial, so the caller's fields are unchanged), applies the fastest and
     returns (best, {shape: ms})."""
     names = ("Conserved", "temperature", "Conserved_t", "Conserved_t_old")

@@ -203,10 +203,12 @@ int htr_chem_source(const htr_grid *g, const htr_fields *f, const htr_mix *m, vo
 int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int precision,
                     const htr_mix *m, const htr_diag *diag, void *stream);
 
-/* Launch shape of the persistent cell loop of htr_chem_update (no reference
- * counterpart: set by the autotuner): threads per block (64, 128, 256) and
- * resident blocks per CU (1..64).  Applies to later calls of every mixture. */
-int htr_chem_launch_config(int block, int blocks_per_cu);
+/* Launch shape of htr_chem_update (no reference counterpart: set by the
+ * autotuner): lanes of a wave that integrate one cell together (8, 16, 32 or
+ * 64, i.e. 8 .. 1 cells per wave; 0 = the default, 64 for 16 or more species,
+ * 16 below) and resident one-wave blocks per CU (1 .. 32; 0 = as many as the
+ * LDS holds).  Applies to later calls of every mixture. */
+int htr_chem_launch_config(int lanes_per_cell, int blocks_per_cu);
 
 /* ---- CFL (src/prometeo_cfl.hpp:55-190) ------------------------------------ */
 /* CalculateMaxSpectralRadiusTask::gpu_base_impl (src/prometeo_cfl.cu:97-152);
