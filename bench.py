@@ -44,10 +44,11 @@ RHS_BYTES_PER_CELL = 495.0
 #                   p (48 B) + 13 B as above, Conserved_t[5:9] RMW (64 B) = 125 B
 #   euler_roe       per TENO face: own-cell Roe inputs (88 B) + SoS (8 B)
 #                   read, face record (22 doubles, 176 B) written = 272 B
-#   euler_char      per (component, TENO face): 1/9 of the record (20 B),
-#                   component + bits written (9 B) = 29 B; FP64-bound, see flops
-#   euler_finish    per TENO face: record 176 B + components 72 B + bits 9 B
-#                   read, face flux 72 B written = 329 B
+#   euler_teno      per TENO face: face record (21 doubles, 168 B) read, the
+#                   limiter test's rho Y_i, old rho Y_i and m_e of the face's two
+#                   cells (22 doubles, 176 B) read, face flux (72 B) written =
+#                   416 B; the stencil points are shared between faces and
+#                   not counted; FP64-bound, see KERNEL_FLOPS
 #   euler_divergence per cell: face flux 72 B + m_e 8 B read, Conserved_t
 #                   read-modify-write 144 B = 224 B
 #   diffusion_species per cell: rho Y 5, T, X 5, rho, lambda, D 5, m_s, m_d
@@ -56,14 +57,18 @@ RHS_BYTES_PER_CELL = 495.0
 #   diffusion_stress per cell: mu, u 3, m_s, m_d and the transverse m_d 2
 #                   (8 doubles) + 3 node types + heat flux read,
 #                   Conserved_t[5:9] RMW (64 B) = 148 B
-#   chem_source     per cell: rho, p, T, Y 5 (64 B) + Conserved_t[0:5] RMW
-#                   (80 B) = 144 B
-KERNEL_BYTES = {"euler_classify": 5.0, "euler_kg_species": 149.0, "euler_kg_flow": 125.0, "euler_roe": 272.0, "euler_char": 29.0, "euler_finish": 329.0,
-                "euler_divergence": 224.0, "diffusion_species": 252.0, "diffusion_stress": 148.0, "chem_source_kernel": 144.0}
+#   chem_source     per cell: rho, p, T, Y 5 (64 B) read, Conserved_t (9
+#                   doubles, 72 B) written = 136 B (the source also clears
+#                   the stage's Conserved_t)
+#   enthalpy        per cell: rhoE, p, rho read, H written = 32 B
+KERNEL_BYTES = {"euler_classify": 5.0, "euler_kg_species": 149.0, "euler_kg_flow": 125.0, "euler_roe": 272.0,
+                "euler_teno": 416.0, "euler_divergence": 224.0, "diffusion_species": 252.0,
+                "diffusion_stress": 148.0, "chem_source_kernel": 136.0, "enthalpy": 32.0}
 # FP64 operations per unit of the compute-bound kernels (hand count of the
-# source: split fluxes of six points 528, two TENO reconstructions ~300, the
-# projection constants ~20)
-KERNEL_FLOPS = {"euler_char": 850.0}
+# source, FMA-free build: per component the species sums and split fluxes of
+# six points ~600, two TENO reconstructions ~300; nine components plus the
+# back-projection and limiter test ~150)
+KERNEL_FLOPS = {"euler_teno": 8250.0}
 
 
 def parse():
@@ -77,13 +82,15 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=48, help="edge of the CPU-baseline box")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the per-cell-composition and boundary-layer readings of the default run")
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
     ap.add_argument("--layout", default="soa", choices=["soa", "aos"],
                     help="vector fields structure-of-arrays, or per point as the reference's VecNEq/VecNSp/Vec3")
     ap.add_argument("--workload", default="air", choices=["air", "bl"],
                     help="air: 256^3 periodic air box; bl: 512x256x128 Mach 6 boundary layer")
-    ap.add_argument("--chem-n", type=int, default=0,
-                    help="edge of the batched-chemistry sample (0: skip; the chemistry line is a separate run)")
+    ap.add_argument("--chem-n", type=int, default=32,
+                    help="edge of the batched-chemistry sample (0: skip)")
     ap.add_argument("--chem-dt", type=float, default=1e-5)
     ap.add_argument("--chem-only", action="store_true",
                     help="only the batched-chemistry measurement (its own JSON line; --chem-n, --chem-dt)")
@@ -146,19 +153,11 @@ def setup_gpu(n, seed, composition="per-seed", layout="soa"):
 
 
 def run_rhs(H, fs, mix, scheme, dt, grid=None):
-    """One full RHS, launched task by task in the reference order (zero,
-    chemistry source, Euler Z Y X, diffusion Z Y X) on torch's current stream."""
-    import ctypes as C
-    lib = H.lib()
-    g = C.byref(grid if grid is not None else fs.grid())
-    f = C.byref(fs.c())
-    s = H.stream_ptr()
-    fs["Conserved_t"].zero_()
-    H.check(lib.htr_chem_source(g, f, mix.h, s))
-    for d in (2, 1, 0):
-        H.check(lib.htr_euler_flux(d, scheme, g, f, 1.0, 0.0, dt, mix.h, None, s))
-    for d in (2, 1, 0):
-        H.check(lib.htr_diffusion_flux(d, g, f, mix.h, s))
+    """One full RHS through the library's one-call entry point (htr_full_rhs:
+    chemistry source clearing Conserved_t, total enthalpy once, Euler Z Y X,
+    diffusion Z Y X, the reference accumulation order) on torch's current
+    stream."""
+    H.full_rhs(fs, scheme, (1.0, 0.0), dt, True, mix, grid=grid)
 
 
 def face_counts(H, fs, mix, scheme, dt, n, grid=None):
@@ -237,10 +236,8 @@ def units_of(fam, name, nfaces_dir, cells, counts):
     teno = counts[d][0] if counts else 0
     if fam == "euler_classify":
         return nfaces_dir[d]
-    if fam in ("euler_roe", "euler_finish"):
+    if fam in ("euler_roe", "euler_teno"):
         return teno
-    if fam == "euler_char":
-        return 9 * teno
     return cells
 
 
@@ -358,43 +355,28 @@ def calibrate_copy():
                       "ms_per_copy": ms, "GBps_read_plus_write": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def main():
-    a = parse()
-    if a.calibrate_copy:
-        calibrate_copy()
-        return
-    if a.chem_only:
-        import htr_amd as H
-        H.init(0)
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out = chemistry(max(a.chem_n, 4), a.chem_dt, a.seed, not a.no_cpu, threads, a.chem_tune)
-        out.update({"metric": "implicit-chemistry solves/s (30 species, 34x34 Jacobians), FP32+IR vs FP64",
-                    "value": out["fp32_ir"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-                    "higher_is_better": True, "dtype": "f64 (FP32 LU + FP64 refinement)", "data": "synthetic"})
-        print(json.dumps(out), flush=True)
-        return
-    if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's share of the "
-              "domain is the unit of work; see the sharding section of DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def measure_rhs(H, a, workload, composition, traffic_table=False):
+    """Sets up the workload, counts its TENO faces, times a.steps full RHS
+    after a.warmup; returns the reading (throughput, per-kernel times, the
+    roofline of the dominant kernel) and frees the fields."""
     import torch
-    import htr_amd as H
-    H.init(0)
     scheme = {"hybrid": H.HTR_EULER_HYBRID, "TENO": H.HTR_EULER_TENO, "KG": H.HTR_EULER_KG}[a.scheme]
     n = a.n
     t_setup = time.perf_counter()
-    if a.workload == "bl":
+    if workload == "bl":
         fs, mix, dt, meta, grid = setup_bl(a.seed)
         cells = 512 * 256 * 128
         nfaces_dir = [(513 * 256 * 128), (512 * 257 * 128), (512 * 256 * 129)]
     else:
-        fs, mix, dt, meta = setup_gpu(n, a.seed, a.composition, a.layout)
+        fs, mix, dt, meta = setup_gpu(n, a.seed, composition, a.layout)
         grid = None
         cells = n ** 3
         nfaces_dir = [(n + 1) * n * n] * 3
-    print("setup %.1fs, %s, dt=%.3e" % (time.perf_counter() - t_setup, a.workload, dt), file=sys.stderr, flush=True)
+    print("setup %.1fs, %s %s, dt=%.3e" % (time.perf_counter() - t_setup, workload, composition, dt),
+          file=sys.stderr, flush=True)
     counts = face_counts(H, fs, mix, scheme, dt, n, grid)
     print("TENO faces per direction %s" % counts, file=sys.stderr, flush=True)
+    H.workspace_reserve(fs, mix, grid=grid)
     print("timing %d RHS steps" % (a.warmup + a.steps), file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         run_rhs(H, fs, mix, scheme, dt, grid)
@@ -420,27 +402,71 @@ def main():
     units = units_of(fam, dom, nfaces_dir, cells, counts) if fam else cells
     launch_ms = per_launch[dom]
     bpu = KERNEL_BYTES.get(fam, 0.0)
-    achieved = bpu * units / (launch_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_unit": bpu, "units_per_launch": units,
-            "launch_ms": launch_ms}
-    default_workload = (a.workload == "air" and n == 256 and a.composition == "per-seed" and a.scheme == "hybrid"
-                        and a.layout == "soa")
-    traffic = measured_traffic(dom) if default_workload else None
+    gbs = bpu * units / (launch_ms * 1e-3) / 1e9
+    if fam in KERNEL_FLOPS:
+        # FP64-bound: the FP64 rate against the 78.6 TFLOP/s FP64 peak (the
+        # MI355X FP64 vector and matrix peaks are the same figure)
+        tf = KERNEL_FLOPS[fam] * units / (launch_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": dom, "achieved": tf, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": tf / FP64_PEAK_TFS, "traffic": None, "flops_per_unit": KERNEL_FLOPS[fam],
+                "units_per_launch": units, "launch_ms": launch_ms,
+                "note": "FP64 VALU-bound (no MFMA); peak = MI355X FP64 78.6 TFLOP/s",
+                "hbm_achieved_GBps": gbs, "bytes_per_unit": bpu}
+    else:
+        roof = {"bound": "hbm", "kernel": dom, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_unit": bpu, "units_per_launch": units,
+                "launch_ms": launch_ms}
+    traffic = measured_traffic(dom) if traffic_table else None
     if traffic is not None:
         roof["traffic"] = traffic
         roof["traffic_source"] = os.path.relpath(TRAFFIC_TABLE, ROOT)
         roof["algorithmic_bytes"] = bpu * units
-    if fam in KERNEL_FLOPS:
-        tf = KERNEL_FLOPS[fam] * units / (launch_ms * 1e-3) / 1e12
-        roof["fp64_tflops"] = tf
-        roof["fp64_frac"] = tf / FP64_PEAK_TFS
     faces = sum(nfaces_dir)
+    out = {"value": cells / (ms_step * 1e-3), "ms_per_step": ms_step, "cells": cells,
+           "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
+           "teno_face_frac": sum(c[0] for c in counts.values()) / faces,
+           "limiter_faces": {d: c[1] for d, c in counts.items()},
+           "rhs_roofline": {"bytes_per_cell": RHS_BYTES_PER_CELL,
+                            "achieved": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9,
+                            "frac": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "roofline": roof}
+    print("RHS %s %s: %.3f ms/step" % (workload, composition, ms_step), file=sys.stderr, flush=True)
+    del fs
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    a = parse()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
+    if a.chem_only:
+        import htr_amd as H
+        H.init(0)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out = chemistry(max(a.chem_n, 4), a.chem_dt, a.seed, not a.no_cpu, threads, a.chem_tune)
+        out.update({"metric": "implicit-chemistry solves/s (30 species, 34x34 Jacobians), FP32+IR vs FP64",
+                    "value": out["fp32_ir"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+                    "higher_is_better": True, "dtype": "f64 (FP32 LU + FP64 refinement)", "data": "synthetic"})
+        print(json.dumps(out), flush=True)
+        return
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's share of the "
+              "domain is the unit of work; see the sharding section of DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import htr_amd as H
+    H.init(0)
+    n = a.n
+    default_workload = (a.workload == "air" and n == 256 and a.composition == "per-seed" and a.scheme == "hybrid"
+                        and a.layout == "soa")
+    main_r = measure_rhs(H, a, a.workload, a.composition, traffic_table=default_workload)
     result = {
         "metric": "RHS cell-updates/s (%s, 5-species air, full FP64 RHS)" % ("256^3 box" if a.workload == "air" else "512x256x128 boundary layer"),
-        "value": cells / (ms_step * 1e-3),
+        "value": main_r["value"],
         "unit": "cell-updates/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": main_r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": ("air5_box_%d^3_rhs_%s_%s" % (n, a.scheme, a.composition)) if a.workload == "air"
@@ -449,21 +475,25 @@ def main():
                    "species": 5, "scheme": a.scheme, "composition": a.composition if a.workload == "air" else None,
                    "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed,
                    "layout": a.layout if a.workload == "air" else "soa"},
-        "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
-        "teno_face_frac": sum(c[0] for c in counts.values()) / faces,
-        "limiter_faces": {d: c[1] for d, c in counts.items()},
-        "rhs_roofline": {"bytes_per_cell": RHS_BYTES_PER_CELL,
-                         "achieved": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9,
-                         "frac": RHS_BYTES_PER_CELL * cells / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "roofline": roof,
     }
-    print("RHS %.3f ms/step" % ms_step, file=sys.stderr, flush=True)
+    for k in ("kernel_ms_per_step", "teno_face_frac", "limiter_faces", "rhs_roofline", "roofline"):
+        result[k] = main_r[k]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if default_workload and not a.no_extras:
+        # the same box with the mass fractions drawn per cell (TENO-heavy)
+        pc = measure_rhs(H, a, "air", "per-cell")
+        result["per_cell_composition"] = {k: pc[k] for k in ("value", "ms_per_step", "teno_face_frac",
+                                                             "rhs_roofline", "roofline", "kernel_ms_per_step")}
+        # the boundary-layer workload (config 4)
+        bl = measure_rhs(H, a, "bl", None)
+        result["boundary_layer"] = {k: bl[k] for k in ("value", "ms_per_step", "teno_face_frac", "rhs_roofline",
+                                                       "roofline")}
     if a.chem_n > 0:
-        result["chemistry"] = chemistry(a.chem_n, a.chem_dt, a.seed, not a.no_cpu, threads)
+        result["chemistry"] = chemistry(a.chem_n, a.chem_dt, a.seed, not a.no_cpu, threads, a.chem_tune)
     if a.serve_requests > 0:
         result["serving"] = serving(a.serve_requests, a.serve_rate, a.seed)
     if not a.no_cpu:
+        scheme = {"hybrid": H.HTR_EULER_HYBRID, "TENO": H.HTR_EULER_TENO, "KG": H.HTR_EULER_KG}[a.scheme]
         rate, reps, el = cpu_baseline(a.cpu_sample, a.seed, scheme, a.cpu_seconds, a.composition)
         result["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                   "sample": "%d^3 box of the same workload, %d full RHS in %.1f s"

@@ -16,9 +16,9 @@ meta holds the mixture parameters and scalars of the case.
                      flat-plate boundary layer, Mach 6 air at 220 K over an
                      isothermal 300 K wall: x periodic (nx), y wall-normal with
                      staggered wall ghosts and tanh stretching (max adjacent
-                     spacing ratio 1.04), z periodic; laminar profile from an
-                     error-function velocity and the Crocco-Busemann
-                     temperature, plus 1 % seeded white noise on the velocity
+                     spacing ratio 1.04), z periodic; the laminar compressible
+                     similarity profile (Chapman-Rubesin parameter 1) plus 1 %
+                     seeded white noise on the velocity
   chem30_box(n, seed) cells of the air box (same seed: p, u, and T mapped
                      linearly from [2500, 7000] K onto [1500, 3000] K, inside
                      the validity range of the 30-species thermodynamic data)
@@ -191,8 +191,48 @@ def stretching_for_ratio(n, ratio, lo=1e-3, hi=6.0):
     return lo
 
 
+def similarity_profile(mach, Tw_over_Tinf, prandtl=0.72, gamma=1.4, eta_max=14.0, n=2801):
+    """Laminar compressible flat-plate boundary layer (similarity solution with
+    Chapman-Rubesin parameter 1, rho mu = const): in the Howarth-Dorodnitsyn
+    variable eta,
+        f''' + f f''/2 = 0,   g'' + Pr f g'/2 + Pr (gamma - 1) M^2 f''^2 = 0,
+        f(0) = f'(0) = 0, f'(inf) = 1,  g(0) = Tw/Tinf, g(inf) = 1,
+    with u/U = f' and T/Tinf = g.  The wall-normal distance in units of
+    sqrt(nu_inf x / U) is Y(eta) = integral of g from 0 to eta (rho ~ 1/T at
+    constant pressure).  Returns (Y, u/U, T/Tinf) on n points."""
+    from scipy.integrate import solve_bvp, cumulative_trapezoid
+    eta = np.linspace(0.0, eta_max, n)
+
+    def rhs(x, y):
+        f, fp, fpp, g, gp = y
+        return np.vstack([fp, fpp, -0.5 * f * fpp, gp,
+                          -0.5 * prandtl * f * gp - prandtl * (gamma - 1.0) * mach ** 2 * fpp ** 2])
+
+    def bc(a, b):
+        return np.array([a[0], a[1], b[1] - 1.0, a[3] - Tw_over_Tinf, b[3] - 1.0])
+
+    guess = np.zeros((5, n))
+    guess[1] = np.tanh(eta / 2.0)
+    guess[0] = 2.0 * np.log(np.cosh(eta / 2.0))
+    guess[2] = 0.5 / np.cosh(eta / 2.0) ** 2
+    guess[3] = 1.0 + (Tw_over_Tinf - 1.0) * np.exp(-eta / 2.0) + 0.5 * (gamma - 1.0) * mach ** 2 * \
+        np.exp(-eta ** 2 / 8.0)
+    sol = solve_bvp(rhs, bc, eta, guess, tol=1e-8, max_nodes=100000)
+    if not sol.success:
+        raise RuntimeError("similarity solution did not converge: " + sol.message)
+    y = sol.sol(eta)
+    Y = cumulative_trapezoid(y[3], eta, initial=0.0)
+    return Y, y[1], y[3]
+
+
 def boundary_layer(nx=512, ny=256, nz=128, seed=0, mach=6.0, T_inf=220.0, T_wall=300.0, p=1.0e4,
                    Lx=0.5, Ly=0.05, Lz=0.1, delta=0.01, noise=0.01):
+    """Flat-plate boundary layer of 5-species air: Mach `mach` freestream at
+    T_inf over an isothermal wall at T_wall (y = 0, staggered ghost below the
+    first cell), wall-normal tanh stretching with largest adjacent-spacing
+    ratio 1.04, x and z periodic; the laminar similarity profile
+    (similarity_profile) scaled to a 99 % thickness `delta`, plus `noise` x U
+    seeded white noise on the velocity of the interior cells."""
     rng = np.random.default_rng(seed)
     s = stretching_for_ratio(ny, 1.04)
     nt, me, md, ms, y = wall_metrics(ny, width=Ly, stretching=s)
@@ -200,12 +240,11 @@ def boundary_layer(nx=512, ny=256, nz=128, seed=0, mach=6.0, T_inf=220.0, T_wall
     gamma = 1.4
     c_inf = math.sqrt(gamma * RGAS / W * T_inf)
     U = mach * c_inf
-    from math import erf
-    eta = y / delta
-    uu = np.array([erf(2.0 * e) for e in eta])                    # u / U
-    r = 0.85 ** 0.5
-    T_aw = T_inf * (1.0 + r * 0.5 * (gamma - 1.0) * mach ** 2)
-    T = T_wall + (T_aw - T_wall) * uu + (T_inf - T_aw) * uu ** 2   # Crocco-Busemann
+    Yb, ub, gb = similarity_profile(mach, T_wall / T_inf, gamma=gamma)
+    Y99 = float(np.interp(0.99, ub, Yb))
+    yb = Yb * (delta / Y99)                                          # physical wall distance
+    uu = np.interp(y, yb, ub, right=1.0)
+    T = T_inf * np.interp(y, yb, gb, right=1.0)
     nyg = ny + 2
     _, _, uRef = air_refs()
     vel = np.zeros((3, nz, nyg, nx))

@@ -21,8 +21,16 @@ A pack's x extent is a fixed capacity handed out in segments (first fit);
 admission requires a free segment and, for a new pack, room in the HBM budget.
 Requests that arrive while others are running are admitted at the next step
 boundary, as segments free up.  Each pack runs on its own stream, so the packs'
-launches overlap on the device.  A case's results do not depend on its
-neighbours: the same case alone and in a mix gives the same fields.
+launches overlap on the device; a pack's library workspace is sized when the
+pack is made, so no step allocates.  A case's results do not depend on its
+neighbours: the same case alone and in a mix gives the same fields, and they
+equal a run of the case on its own box (the oracle test).
+
+Per RK stage a pack issues: the Ducros and shock sensors, the chemistry
+source (5 species), the Euler and diffusion sweeps of its directions, the
+fused stage update (update + primitives + properties in one pass,
+htr_rk_stage) and one index copy per field that refreshes every ghost and
+halo cell of every resident case at once.
 """
 import time
 
@@ -73,6 +81,12 @@ def mixture(kind):
     return A.Mixture.airmix(meta["PRef"], meta["TRef"], meta["LRef"], meta["xref"])
 
 
+# per-cell state fields a ghost or halo cell carries (metrics and node types
+# are static; Conserved_old and Conserved_t are stage temporaries)
+STATE = ("Conserved", "temperature", "pressure", "MolarFracs", "velocity", "rho", "mu", "lam", "Di", "SoS",
+         "MassFracs")
+
+
 class Pack:
     """Resident cases of one class (kind, ny) side by side along x."""
 
@@ -85,10 +99,15 @@ class Pack:
         self.dirs = (0, 1) if kind == "vortex" else (0,)
         self.free = [(0, capacity)]                 # (start, length), sorted
         self.resident = {}                          # id -> (request, x0)
-        self.ghost = {}                             # id -> saved ghost Conserved (tubes)
+        self.ghost = {}                             # id -> (flat indices, saved state) of a tube's two ghosts
         self.stream = torch.cuda.Stream() if device == "cuda" else None
+        self._bnd = None                            # boundary copy plan (rebuilt on admission/release)
         # pad cells: a quiet state so that unused columns stay finite
         self._fill_pad(0, capacity)
+        if self.stream is not None:
+            # the library workspace of this stream, sized once for the pack
+            A.workspace_reserve(self.fs, self.mix, stream=self.stream)
+            torch.cuda.synchronize()
 
     def nbytes(self):
         return sum(t.numel() * t.element_size() for t in self.fs.t.values())
@@ -120,6 +139,9 @@ class Pack:
                 return True
         return False
 
+    def _flat(self, x, y):
+        return np.asarray(y) * self.cap + np.asarray(x)
+
     def _place(self, req, x0):
         st, _ = req.state()
         w = req.width()
@@ -131,33 +153,61 @@ class Pack:
         g = self._grid(x0, w)
         if self.kind == "vortex":
             # halos: periodic images of every field, metrics and node types included
-            self._halo_fields(x0, req.n, tuple(self.fs.t))
+            a, b = x0 + HALO, x0 + HALO + req.n
+            for k in self.fs.t:
+                t = self.fs[k]
+                t[..., a - HALO:a] = t[..., b - HALO:b]
+                t[..., b:b + HALO] = t[..., a:a + HALO]
         A.update_properties_from_primitive(self.fs, self.mix, grid=g)
         A.update_conserved_from_primitive(self.fs, self.mix, grid=g)
         if self.kind != "vortex":
-            self.ghost[req.id] = (self.fs["Conserved"][..., x0].clone(), self.fs["Conserved"][..., x0 + w - 1].clone())
+            idx = torch.as_tensor(self._flat([x0, x0 + w - 1], [0, 0]), device=self.fs["rho"].device)
+            self.ghost[req.id] = (idx, {k: self._cols(k)[:, idx].clone() for k in STATE})
         self.resident[req.id] = (req, x0)
+        self._bnd = None
 
-    def _halo_fields(self, x0, n, names):
-        a, b = x0 + HALO, x0 + HALO + n
-        for k in names:
-            t = self.fs[k]
-            t[..., a - HALO:a] = t[..., b - HALO:b]
-            t[..., b:b + HALO] = t[..., a:a + HALO]
+    def _cols(self, k):
+        """Field k as (components, cells) over the pack's plane."""
+        t = self.fs[k]
+        return t.reshape(-1, self.cap * self.ny) if t.dim() == 4 else t.reshape(1, -1)
+
+    def _plan(self):
+        """Flat indices of every ghost and halo cell of the resident cases and
+        of the cells their values come from (vortex halos: the periodic
+        images; tube ghosts: the saved Dirichlet states)."""
+        dev = self.fs["rho"].device
+        if self.kind == "vortex":
+            dst, src = [], []
+            ys = np.arange(self.ny)
+            for req, x0 in self.resident.values():
+                a, b = x0 + HALO, x0 + HALO + req.n
+                for h in range(HALO):
+                    dst.append(self._flat(a - HALO + h, ys)); src.append(self._flat(b - HALO + h, ys))
+                    dst.append(self._flat(b + h, ys)); src.append(self._flat(a + h, ys))
+            dst = torch.as_tensor(np.concatenate(dst), device=dev)
+            src = torch.as_tensor(np.concatenate(src), device=dev)
+            return dst, src, None
+        ids = list(self.ghost)
+        dst = torch.cat([self.ghost[r][0] for r in ids])
+        saved = {k: torch.cat([self.ghost[r][1][k] for r in ids], dim=1) for k in STATE}
+        return dst, None, saved
 
     def _boundaries(self):
-        """Dirichlet ghosts of the tubes, periodic halos of the vortices."""
-        for rid, (req, x0) in self.resident.items():
-            if self.kind == "vortex":
-                self._halo_fields(x0, req.n, ("Conserved", "temperature"))
-            else:
-                gl, gr = self.ghost[rid]
-                self.fs["Conserved"][..., x0] = gl
-                self.fs["Conserved"][..., x0 + req.width() - 1] = gr
+        """Dirichlet ghosts of the tubes, periodic halos of the vortices: one
+        index copy per state field for the whole pack."""
+        if not self.resident:
+            return
+        if self._bnd is None:
+            self._bnd = self._plan()
+        dst, src, saved = self._bnd
+        for k in STATE:
+            c = self._cols(k)
+            c[:, dst] = c[:, src] if src is not None else saved[k]
 
     def release(self, rid):
         req, x0 = self.resident.pop(rid)
         self.ghost.pop(rid, None)
+        self._bnd = None
         w = req.width()
         self._fill_pad(x0, w)
         self.free.append((x0, w))
@@ -177,18 +227,16 @@ class Pack:
         return self.fs["Conserved"][..., x0 + 1:x0 + 1 + req.n].cpu().numpy().copy()
 
     def step(self):
-        """One SSP-RK3 step of every resident case (src/prometeo.rg:428-478):
-        per stage primitive recompute, boundary values, properties, sensors,
-        RHS, stage update; one launch per kernel for the whole pack."""
+        """One SSP-RK3 step of every resident case (src/prometeo.rg:1407-1470):
+        per stage the sensors, the RHS with the stage's deltaTime
+        dt RK_C[stage][3], the fused update + primitives + properties, then
+        the boundary values; one launch per kernel for the whole pack."""
         fs, mix, dt = self.fs, self.mix, self.dt
         neq = self.nspec + 4
         ctx = torch.cuda.stream(self.stream) if self.stream is not None else _null()
         with ctx:
             A.rk_init(fs, neq)
             for stage in range(3):
-                self._boundaries()
-                A.update_primitive_from_conserved(fs, mix)
-                A.update_properties_from_primitive(fs, mix)
                 A.update_ducros_sensor(fs, 1.0)
                 for d in self.dirs:
                     A.update_shock_sensor(fs, d, mix)
@@ -196,12 +244,13 @@ class Pack:
                 if self.nspec > 1:
                     A.add_chemistry_sources(fs, mix)
                 RK = ((1.0, 0.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0))[stage]
+                c3 = (1.0, 0.25, 2.0 / 3.0)[stage]
                 for d in reversed(self.dirs):
-                    A.update_using_euler_flux(fs, d, A.HTR_EULER_HYBRID, RK, dt, mix)
+                    A.update_using_euler_flux(fs, d, A.HTR_EULER_HYBRID, RK, dt * c3, mix)
                 for d in reversed(self.dirs):
                     A.update_using_diffusion_flux(fs, d, mix)
-                A.rk_update(fs, neq, stage, dt)
-            self._boundaries()
+                A.rk_stage(fs, stage, mix, deltaTime=dt)
+                self._boundaries()
 
 
 class _null:

@@ -554,3 +554,65 @@ def test_failure_flags():
     others[1, 2, 3] = 0
     others[4, 4, 4] = 0
     assert np.all(others == 0)
+
+
+def test_boundary_layer_reduced():
+    """The boundary-layer workload (Mach 6 air over an isothermal 300 K wall,
+    similarity profile + 1 % noise, tanh-stretched wall-normal grid with
+    staggered ghosts) at 32 x 32 x 16: full RHS (chemistry source, hybrid
+    Euler ZYX, diffusion ZYX) of the interior cells against the oracle."""
+    import htr_amd as H
+    st, meta = cases.boundary_layer(32, 32, 16, seed=1)
+    n = (32, 34, 16)
+    say("case boundary layer 32x34x16 full RHS")
+    st, om = complete_state(st, meta, n)
+    dt = dt_cfl(st, meta, n) * 0.5
+    lo, hi = (0, 1, 0), (31, 32, 15)
+    g, f = oracle_fields(st, 5, n)
+    g = g.with_rect(lo, hi)
+    O.full_rhs(g, f, O.SCHEME_HYBRID, 1.0, 0.0, dt, 1, om)
+    fs = to_gpu(st, 5, n)
+    H.full_rhs(fs, 0, (1.0, 0.0), dt, True, gpu_mix(meta), grid=fs.grid(lo, hi))
+    got = host(fs["Conserved_t"])
+    ref = f["Conserved_t"]
+    e = norm_err(got[:, :, 1:33, :], ref[:, :, 1:33, :])
+    say("  boundary layer full RHS: %s" % ["%.2e" % x for x in e])
+    assert max(e) <= TOL
+
+
+def test_boundary_layer_full_size_properties():
+    """The 512 x 256 x 128 boundary layer on the GPU: every Euler and
+    diffusion sweep gives finite values, and the x and z sweeps (periodic)
+    telescope: the sum over a periodic line of Conserved_t / m_e vanishes to
+    round-off."""
+    import htr_amd as H
+    st, meta = cases.boundary_layer(512, 256, 128, seed=0)
+    n = (512, 258, 128)
+    say("case boundary layer 512x258x128 properties")
+    fs = to_gpu({k: st[k] for k in st}, 5, n)
+    mix = gpu_mix(meta)
+    H.update_properties_from_primitive(fs, mix)
+    H.update_conserved_from_primitive(fs, mix)
+    fs["Conserved_old"].copy_(fs["Conserved"])
+    H.update_ducros_sensor(fs, meta["vorticityScale"])
+    for d in range(3):
+        H.update_shock_sensor(fs, d, mix)
+    grid = fs.grid((0, 1, 0), (511, 256, 127))
+    dt = 0.5 / H.calculate_max_spectral_radius(fs, mix, grid=grid)
+    for d, kind in ((0, "euler"), (2, "euler"), (0, "diffusion"), (2, "diffusion"), (1, "euler"), (1, "diffusion")):
+        fs["Conserved_t"].zero_()
+        if kind == "euler":
+            H.update_using_euler_flux(fs, d, H.HTR_EULER_HYBRID, (1.0, 0.0), dt, mix, grid=grid)
+        else:
+            H.update_using_diffusion_flux(fs, d, mix, grid=grid)
+        ct = fs["Conserved_t"][:, :, 1:257, :]
+        assert bool(torch.isfinite(ct).all()), (kind, d)
+        if d != 1:
+            m = fs["m_e%d" % d if kind == "euler" else "m_d%d" % d][:, 1:257, :]
+            q = ct / m[None]
+            axis = 3 if d == 0 else 1
+            tot = q.sum(dim=axis).abs().max().item()
+            scale = q.abs().amax(dim=axis).max().item() * q.shape[axis]
+            say("  %s dir %d: |sum over periodic lines| %.3e of %.3e" % (kind, d, tot, scale))
+            assert tot <= 1e-12 * scale
+    torch.cuda.synchronize()

@@ -226,3 +226,15 @@ def test_teno_reproduces_quadratics():
                 # stencil reproduces q there exactly
                 exact = a - c / 12.0 + 0.5 * b + 0.25 * c
                 assert abs(v - exact) <= 1e-12 * max(1.0, abs(exact)), (op, side, v, exact)
+
+
+def test_similarity_profile_blasius_limit():
+    """The compressible similarity solution of the boundary-layer workload
+    reduces to Blasius at Mach 0 with an adiabatic-free wall at the
+    freestream temperature: f''(0) = 0.33206 (u/U ~ 0.33206 Y near the wall)
+    and the 99 % thickness Y99 = 4.91 in units of sqrt(nu x / U)."""
+    Y, u, g = cases.similarity_profile(0.0, 1.0)
+    assert np.max(np.abs(g - 1.0)) < 1e-10
+    slope = (u[1] - u[0]) / (Y[1] - Y[0])
+    assert abs(slope - 0.33206) < 2e-4, slope
+    assert abs(float(np.interp(0.99, u, Y)) - 4.91) < 0.02
