@@ -381,7 +381,9 @@ static int layout_views(const htr_grid *g, htr_fields &out, htr_view *extra, int
       {&out.Conserved_t_old, 8, "Conserved_t_old"}, {&out.DucrosSensor, 8, "DucrosSensor"},
       {&out.nType[0], 4, "nType[0]"}, {&out.nType[1], 4, "nType[1]"}, {&out.nType[2], 4, "nType[2]"},
       {&out.shockSensor[0], 1, "shockSensor[0]"}, {&out.shockSensor[1], 1, "shockSensor[1]"},
-      {&out.shockSensor[2], 1, "shockSensor[2]"} };
+      {&out.shockSensor[2], 1, "shockSensor[2]"},
+      {&out.MolarFracs_profile, 8, "MolarFracs_profile"}, {&out.velocity_profile, 8, "velocity_profile"},
+      {&out.temperature_profile, 8, "temperature_profile"} };
    for (const Named &n : all) {
       const int rc = L.add(*n.v, n.esize, n.name, blo);
       if (rc) return rc;
@@ -680,6 +682,28 @@ int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
    if (scheme != HTR_EULER_KG) { HTR_NEED(SoS); if (m->host.nSpec > 1) HTR_NEED(Conserved_old); }
    if (scheme == HTR_EULER_HYBRID) HTR_NEED(shockSensor[dir]);
    HTR_DISPATCH3(euler_dispatch, dir, scheme, fr, m->dev, geo, RK0, RK1, host_dt(deltaTime), dec, (hipStream_t)stream);
+}
+
+int htr_set_bc(int kind, int side, const htr_grid *g, const htr_fields *f, double Pbc, const htr_mix *m,
+               void *stream)
+{
+   if (kind < HTR_BC_DIRICHLET || kind > HTR_BC_ISOTHERMAL_WALL) return set_error(HTR_E_ARG, "bad BC kind %d", kind);
+   if (side < 0 || side > 5) return set_error(HTR_E_ARG, "bad BC side %d", side);
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
+   const int dir = side/2, inward = (side % 2 == 0) ? 1 : -1;
+   // the interior neighbours of the ghost rect lie inside the bounds
+   if ((inward > 0 && g->exec.hi[dir] + 1 > g->bounds.hi[dir]) || (inward < 0 && g->exec.lo[dir] - 1 < g->bounds.lo[dir]))
+      return set_error(HTR_E_ARG, "BC: the interior neighbours of the ghost rect are outside the bounds");
+   HTR_NEED(MolarFracs); HTR_NEED(velocity); HTR_NEED(temperature); HTR_NEED(pressure);
+   if (kind == HTR_BC_DIRICHLET) { HTR_NEED(MolarFracs_profile); HTR_NEED(velocity_profile); }
+   if (kind != HTR_BC_ADIABATIC_WALL) HTR_NEED(temperature_profile);
+   hipStream_t s = (hipStream_t)stream;
+   switch (kind_of(m)) {
+      case K_CP:  return bc_launch<MixCP>(fr, geo, kind, dir, inward, Pbc, s);
+      case K_AIR: return bc_launch<MixAir>(fr, geo, kind, dir, inward, Pbc, s);
+      case K_30:  return bc_launch<Mix30>(fr, geo, kind, dir, inward, Pbc, s);
+      default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
+   }
 }
 
 int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)

@@ -62,6 +62,8 @@ template<class MX> int spectral_radius_launch(const htr_fields &f, const DMix *m
 int chem_set_launch(int block, int blocks_per_cu);
 int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, double dt, hipStream_t s);
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);
+template<class MX> int bc_launch(const htr_fields &f, const Geo &g, int kind, int dir, int inward, double Pbc,
+                                 hipStream_t s);
 template<class MX> int rk_stage_launch(const htr_fields &f, const DMix *mix, const Geo &g, int stage, const DtArg &dt,
                                        const htr_view &flags, hipStream_t s);
 int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);

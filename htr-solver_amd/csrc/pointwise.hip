@@ -356,6 +356,37 @@ int zero_counters(int *counter, hipStream_t s)
    return check_launch("zero_counters_kernel");
 }
 
+// -------------------------------------------------------- boundary conditions
+// Ghost-cell primitives of one boundary (src/prometeo_bc.rg:1159-1265):
+// kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC.  The
+// interior neighbour q of ghost p is one cell along dir, inward (+1 for a
+// Neg side, -1 for a Pos side); the host checks that it lies inside
+// Fluid_bounds.
+template<class MX>
+__global__ void __launch_bounds__(256)
+bc_kernel(const htr_fields f, const Geo g, int kind, int dir, int inward, double Pbc, int64_t n)
+{
+   HTR_POINT_LOOP(g, n)
+   constexpr int NS = MX::NS;
+   P3 q = p;
+   setc(g, q, dir, p.c[dir] + inward);
+   if (kind == 0) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) at<double>(f.MolarFracs, p, i) = ld<double>(f.MolarFracs_profile, p, i);
+#pragma unroll
+      for (int i = 0; i < 3; i++) at<double>(f.velocity, p, i) = ld<double>(f.velocity_profile, p, i);
+      at<double>(f.temperature, p) = ld<double>(f.temperature_profile, p);
+      at<double>(f.pressure, p) = Pbc;
+      return;
+   }
+#pragma unroll
+   for (int i = 0; i < NS; i++) at<double>(f.MolarFracs, p, i) = ld<double>(f.MolarFracs, q, i);
+   at<double>(f.pressure, p) = ld<double>(f.pressure, q);
+   at<double>(f.temperature, p) = (kind == 1) ? ld<double>(f.temperature, q) : ld<double>(f.temperature_profile, p);
+#pragma unroll
+   for (int i = 0; i < 3; i++) at<double>(f.velocity, p, i) = 0.0;
+}
+
 // ------------------------------------------------------------------ launchers
 #define HTR_LAUNCH_POINT(kern, ...)                                                       \
    const int64_t n = exec_cells(g);                                                       \
@@ -405,6 +436,10 @@ int rk_stage_launch(const htr_fields &f, const DMix *mix, const Geo &g, int stag
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
 { HTR_LAUNCH_POINT(rk_init_kernel, f, g, nEq) }
 
+template<class MX>
+int bc_launch(const htr_fields &f, const Geo &g, int kind, int dir, int inward, double Pbc, hipStream_t s)
+{ HTR_LAUNCH_POINT(bc_kernel<MX>, f, g, kind, dir, inward, Pbc) }
+
 int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
 { HTR_LAUNCH_POINT(zero_rhs_kernel, f, g, nEq) }
 
@@ -437,7 +472,8 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    template int shock_sensor_launch<MX>(int, const htr_fields &, const Geo &, hipStream_t);         \
    template int spectral_radius_launch<MX>(const htr_fields &, const DMix *, const Geo &, double *, hipStream_t); \
    template int rk_stage_launch<MX>(const htr_fields &, const DMix *, const Geo &, int, const DtArg &,           \
-                                    const htr_view &, hipStream_t);
+                                    const htr_view &, hipStream_t);                                               \
+   template int bc_launch<MX>(const htr_fields &, const Geo &, int, int, int, double, hipStream_t);
 HTR_INST_POINT(MixCP)
 HTR_INST_POINT(MixAir)
 HTR_INST_POINT(Mix30)

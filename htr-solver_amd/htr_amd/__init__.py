@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("HTR_AMD_LIBRARY") or os.path.join(PKG, "libhtr_amd.so
 
 HTR_EULER_HYBRID, HTR_EULER_TENO, HTR_EULER_TENOA, HTR_EULER_TENOLAD, HTR_EULER_KG = range(5)
 HTR_CHEM_FP64, HTR_CHEM_FP32_IR = 0, 1
+HTR_BC_DIRICHLET, HTR_BC_ADIABATIC_WALL, HTR_BC_ISOTHERMAL_WALL = 0, 1, 2
 ERRORS = {-1: "HTR_E_ARG", -2: "HTR_E_UNSUPPORTED", -3: "HTR_E_DEVICE", -4: "HTR_E_FIELD", -5: "HTR_E_NUMERIC"}
 
 
@@ -41,7 +42,8 @@ class Fields(C.Structure):
                 ("rho", View), ("mu", View), ("lam", View), ("Di", View), ("SoS", View),
                 ("pressure", View), ("temperature", View), ("MassFracs", View), ("MolarFracs", View),
                 ("velocity", View), ("Conserved", View), ("Conserved_old", View), ("Conserved_t", View),
-                ("Conserved_t_old", View), ("shockSensor", View * 3), ("DucrosSensor", View)]
+                ("Conserved_t_old", View), ("shockSensor", View * 3), ("DucrosSensor", View),
+                ("MolarFracs_profile", View), ("velocity_profile", View), ("temperature_profile", View)]
 
 
 class Diag(C.Structure):
@@ -57,7 +59,7 @@ EXPORTS = [
     "htr_chem_source", "htr_chem_update", "htr_max_spectral_radius",
     "htr_rk_update", "htr_rk_init", "htr_full_rhs",
     "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
-    "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl",
+    "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl", "htr_set_bc",
 ]
 
 _lib = None
@@ -87,6 +89,7 @@ def _declare(L):
         "htr_workspace_reserve": (i, [G, F, vp, vp]),
         "htr_rk_stage": (i, [G, F, i, d, d, vp, vp, D, vp]),
         "htr_full_rhs_cfl": (i, [G, F, i, d, d, d, d, vp, i, vp, vp]),
+        "htr_set_bc": (i, [i, i, G, F, d, vp, vp]),
         "htr_mix_create_constprop": (i, [C.POINTER(vp), d, d, d, i, d, d, d]),
         "htr_mix_create_airmix": (i, [C.POINTER(vp), d, d, d, i, ip, dp]),
         "htr_mix_create_species": (i, [C.POINTER(vp), i, dp, fp, ip, dp, dp, dp, ip, dp]),

@@ -21,7 +21,8 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "update_using_skew_symmetric_euler_flux", "update_using_diffusion_flux",
            "add_chemistry_sources", "update_chemistry", "calculate_max_spectral_radius",
            "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report",
-           "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve"]
+           "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve", "set_bc",
+           "BC_SIDES"]
 
 
 def stream_ptr(stream=None):
@@ -115,6 +116,7 @@ def make_grid(n, lo=(0, 0, 0), hi=None, bounds_lo=(0, 0, 0)):
 SCALARS = ("rho", "mu", "lam", "SoS", "pressure", "temperature", "DucrosSensor")
 VECTORS = {"Di": "ns", "MassFracs": "ns", "MolarFracs": "ns", "velocity": 3,
            "Conserved": "neq", "Conserved_old": "neq", "Conserved_t": "neq", "Conserved_t_old": "neq"}
+PROFILES = {"MolarFracs_profile": "ns", "velocity_profile": 3, "temperature_profile": 0}
 
 
 class FieldSet:
@@ -125,7 +127,7 @@ class FieldSet:
     and presented through a permuted view."""
 
     def __init__(self, n, nspec, device="cuda", bounds_lo=(0, 0, 0), decisions=False, nsteps=False,
-                 layout="soa", flags=False):
+                 layout="soa", flags=False, profiles=False):
         nx, ny, nz = n
         self.n = tuple(n)
         self.ns = nspec
@@ -151,6 +153,11 @@ class FieldSet:
             self.t["nsteps"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
         if flags:
             self.t["flags"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+        if profiles:
+            for k, c in PROFILES.items():
+                nc = {"ns": nspec}.get(c, c)
+                shape = (nz, ny, nx) if nc == 0 else (nc, nz, ny, nx)
+                self.t[k] = torch.zeros(shape, dtype=torch.float64, **kw)
         self._c = None
 
     def __getitem__(self, k):
@@ -185,6 +192,9 @@ class FieldSet:
             f.shockSensor[d] = self.view("shockSensor%d" % d)
         for k in SCALARS + tuple(VECTORS):
             setattr(f, k, self.view(k))
+        for k in PROFILES:
+            if k in self.t:
+                setattr(f, k, self.view(k))
         return f
 
     def diag(self):
@@ -292,6 +302,17 @@ def rk_stage(fs, stage, mix, deltaTime=0.0, cfl=0.0, radius=None, grid=None, str
     rp = C.c_void_p(radius.data_ptr()) if radius is not None else None
     check(lib().htr_rk_stage(_g(fs, grid), C.byref(fs.c()), stage, deltaTime, cfl, rp, mix.h, dg,
                              stream_ptr(stream)))
+
+
+BC_SIDES = ("xNeg", "xPos", "yNeg", "yPos", "zNeg", "zPos")
+
+
+def set_bc(fs, kind, side, grid, mix, Pbc=0.0, stream=None):
+    """Ghost-cell primitives of one boundary (SetDirichletBC /
+    SetAdiabaticWallBC / SetIsothermalWallBC); side is an index or a name of
+    BC_SIDES; grid's execution rect is the boundary's ghost cells."""
+    sd = BC_SIDES.index(side) if isinstance(side, str) else int(side)
+    check(lib().htr_set_bc(int(kind), sd, C.byref(grid), C.byref(fs.c()), float(Pbc), mix.h, stream_ptr(stream)))
 
 
 def workspace_reserve(fs, mix, grid=None, stream=None):

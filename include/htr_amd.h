@@ -94,6 +94,8 @@ typedef struct {
    htr_view Conserved, Conserved_old, Conserved_t, Conserved_t_old;
    htr_view shockSensor[3];    /* shockSensorX/Y/Z */
    htr_view DucrosSensor;
+   /* boundary profiles read by the ghost-cell tasks (src/prometeo_types.h ProfilesVars) */
+   htr_view MolarFracs_profile, velocity_profile, temperature_profile;
 } htr_fields;
 
 /* Optional diagnostics written by instrumented calls (any view may be null):
@@ -179,6 +181,20 @@ int htr_update_primitive(const htr_grid *g, const htr_fields *f, const htr_mix *
 int htr_ducros_sensor(const htr_grid *g, const htr_fields *f, double vorticityScale, void *stream);
 /* UpdateShockSensorTask<dir>::gpu_base_impl (src/prometeo_sensor.cu:121-220) */
 int htr_shock_sensor(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+
+/* ---- boundary conditions (src/prometeo_bc.rg:1159-1265) -------------------- */
+#define HTR_BC_DIRICHLET       0   /* SetDirichletBC: X, u, T from the profiles, p = Pbc */
+#define HTR_BC_ADIABATIC_WALL  1   /* SetAdiabaticWallBC: X, p, T of the interior neighbour, u = 0 */
+#define HTR_BC_ISOTHERMAL_WALL 2   /* SetIsothermalWallBC: X, p of the neighbour, T from the profile, u = 0 */
+/* Primitive variables of the ghost cells of one boundary, the BC part of
+ * UpdateGhostPrimitives: g->exec is the ghost rect of the boundary, side
+ * 0..5 = xNeg, xPos, yNeg, yPos, zNeg, zPos names the boundary (the interior
+ * neighbour of a ghost is one cell inward along the normal, and must lie in
+ * Fluid_bounds).  As in the reference, properties and conserved variables of
+ * the ghosts follow with htr_update_properties and htr_update_conserved over
+ * the same rect. */
+int htr_set_bc(int kind, int side, const htr_grid *g, const htr_fields *f, double Pbc, const htr_mix *m,
+               void *stream);
 
 /* ---- Euler fluxes (src/prometeo_rhs.hpp:171-366) -------------------------- */
 /* scheme HTR_EULER_HYBRID: UpdateUsingHybridEulerFluxTask<dir>::gpu_base_impl

@@ -120,6 +120,7 @@ typedef struct {
    double *Conserved, *Conserved_old, *Conserved_t, *Conserved_t_old;
    uint8_t *shockSensor[3];
    double *DucrosSensor;
+   double *MolarFracs_profile, *velocity_profile, *temperature_profile;
 } o_fields;
 
 /* ---- mixture builders --------------------------------------------------- */
@@ -210,6 +211,10 @@ void   o_chem_source(const o_grid *g, const o_fields *f, const o_mix *m);
 /* nsteps (may be NULL): Rosenbrock steps per cell */
 void   o_chem_update(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps);
 double o_max_spectral_radius(const o_grid *g, const o_fields *f, const o_mix *m);
+/* ghost-cell primitives of one boundary (src/prometeo_bc.rg:1159-1265):
+ * kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC; side
+ * 0..5 = xNeg, xPos, yNeg, yPos, zNeg, zPos; the rect of g = the ghost cells */
+void   o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc, const o_mix *m);
 /* SSP-RK3 update of Conserved (src/prometeo_rk.rg:88-112) */
 void   o_rk_update(const o_grid *g, const o_fields *f, int nEq, int stage, double dt);
 

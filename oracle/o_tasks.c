@@ -595,3 +595,37 @@ void o_full_rhs(const o_grid *g, const o_fields *f, int scheme, double RK0, doub
    for (int d = 2; d >= 0; d--) o_euler_flux(g, f, d, scheme, RK0, RK1, deltaTime, m, 0);
    for (int d = 2; d >= 0; d--) o_diffusion_flux(g, f, d, m);
 }
+
+/* ------------------------------------------------------------------------
+ * Boundary ghost cells (src/prometeo_bc.rg:1159-1265): SetDirichletBC,
+ * SetAdiabaticWallBC, SetIsothermalWallBC.  The interior neighbour of a
+ * ghost is c_int = c + 1 along the normal for a Neg side, c - 1 for a Pos
+ * side (mk_cint of the reference).
+ * Pinned by tests/test_oracle_pins.py (wall and Dirichlet ghost values,
+ * zero mass flux through a wall).
+ */
+void o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc, const o_mix *m)
+{
+   const int ns = m->nSpec;
+   const int dir = side/2, inward = (side % 2 == 0) ? 1 : -1;
+   const long N = (long)g->nx*g->ny*g->nz;
+   for (int k = g->lo[2]; k <= g->hi[2]; k++)
+      for (int j = g->lo[1]; j <= g->hi[1]; j++)
+         for (int i = g->lo[0]; i <= g->hi[0]; i++) {
+            int c[3] = { i, j, k };
+            const long p = (long)i + (long)g->nx*((long)j + (long)g->ny*(long)k);
+            c[dir] += inward;
+            const long q = (long)c[0] + (long)g->nx*((long)c[1] + (long)g->ny*(long)c[2]);
+            if (kind == 0) {
+               for (int s = 0; s < ns; s++) f->MolarFracs[s*N + p] = f->MolarFracs_profile[s*N + p];
+               for (int s = 0; s < 3; s++) f->velocity[s*N + p] = f->velocity_profile[s*N + p];
+               f->temperature[p] = f->temperature_profile[p];
+               f->pressure[p] = Pbc;
+            } else {
+               for (int s = 0; s < ns; s++) f->MolarFracs[s*N + p] = f->MolarFracs[s*N + q];
+               f->pressure[p] = f->pressure[q];
+               f->temperature[p] = (kind == 1) ? f->temperature[q] : f->temperature_profile[p];
+               for (int s = 0; s < 3; s++) f->velocity[s*N + p] = 0.0;
+            }
+         }
+}

@@ -56,7 +56,8 @@ class OFields(C.Structure):
                 ("MolarFracs", _dp), ("velocity", _dp),
                 ("Conserved", _dp), ("Conserved_old", _dp), ("Conserved_t", _dp),
                 ("Conserved_t_old", _dp),
-                ("shockSensor", _u8p * 3), ("DucrosSensor", _dp)]
+                ("shockSensor", _u8p * 3), ("DucrosSensor", _dp),
+                ("MolarFracs_profile", _dp), ("velocity_profile", _dp), ("temperature_profile", _dp)]
 
 
 def _declare(L):
@@ -119,6 +120,7 @@ def _declare(L):
         "o_chem_update": (None, [C.POINTER(OGrid), C.POINTER(OFields), d, vp, C.POINTER(C.c_int32)]),
         "o_max_spectral_radius": (d, [C.POINTER(OGrid), C.POINTER(OFields), vp]),
         "o_rk_update": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, i, d]),
+        "o_set_bc": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, i, d, vp]),
         "o_full_rhs": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, d, d, d, i, vp]),
         "o_lu_solve": (None, [i, _dp, _dp]),
         "o_rosenbrock_testprob": (None, [_dp, d, d, C.POINTER(C.c_int32)]),
@@ -292,6 +294,7 @@ FIELD_COMPONENTS = {
     "rho": 0, "mu": 0, "lam": 0, "SoS": 0, "pressure": 0, "temperature": 0, "DucrosSensor": 0,
     "Di": "ns", "MassFracs": "ns", "MolarFracs": "ns", "velocity": 3,
     "Conserved": "neq", "Conserved_old": "neq", "Conserved_t": "neq", "Conserved_t_old": "neq",
+    "MolarFracs_profile": "ns", "velocity_profile": 3, "temperature_profile": 0,
 }
 
 
@@ -382,6 +385,10 @@ def chem_update(g, f, dt, m, nsteps=None):
 
 def max_spectral_radius(g, f, m):
     return lib().o_max_spectral_radius(C.byref(g.c()), C.byref(f.c()), m.ptr)
+
+
+def set_bc(g, f, kind, side, Pbc, m):
+    lib().o_set_bc(C.byref(g.c()), C.byref(f.c()), kind, side, Pbc, m.ptr)
 
 
 def rk_update(g, f, nEq, stage, dt):

@@ -616,3 +616,41 @@ def test_boundary_layer_full_size_properties():
             say("  %s dir %d: |sum over periodic lines| %.3e of %.3e" % (kind, d, tot, scale))
             assert tot <= 1e-12 * scale
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("side", range(6))
+def test_bc_ghosts(side):
+    """Ghost-cell primitives of one boundary (Dirichlet, adiabatic and
+    isothermal wall), then properties and conserved variables over the ghost
+    layer, against the oracle (bit-identical copies; properties to 1e-12)."""
+    import htr_amd as H
+    n = (8, 9, 7)
+    st, meta = air_state(9, seed=2)
+    st = {k: (v[..., :n[2], :n[1], :n[0]] if v.ndim >= 3 else v) for k, v in st.items()}
+    st, om = complete_state(st, meta, n, sensors=False)
+    rng = np.random.default_rng(10 + side)
+    prof = {"MolarFracs_profile": st["MolarFracs"][:, ::-1].copy(),
+            "velocity_profile": rng.uniform(-0.5, 0.5, size=st["velocity"].shape),
+            "temperature_profile": st["temperature"][::-1].copy() * 1.1}
+    d, neg = side // 2, side % 2 == 0
+    lo, hi = [0, 0, 0], [n[0] - 1, n[1] - 1, n[2] - 1]
+    lo[d] = hi[d] = 0 if neg else n[d] - 1
+    say("case BC side %d" % side)
+    gm = gpu_mix(meta)
+    for kind in (0, 1, 2):
+        g, f = oracle_fields(dict(st, **prof), 5, n)
+        gr = g.with_rect(tuple(lo), tuple(hi))
+        O.set_bc(gr, f, kind, side, 0.9, om)
+        O.update_properties(gr, f, om)
+        O.update_conserved(gr, f, om)
+        fs = to_gpu(dict(st, **prof), 5, n, profiles=True)
+        grid = fs.grid(tuple(lo), tuple(hi))
+        H.set_bc(fs, kind, side, grid, gm, Pbc=0.9)
+        H.update_properties_from_primitive(fs, gm, grid=grid)
+        H.update_conserved_from_primitive(fs, gm, grid=grid)
+        for k in ("MolarFracs", "velocity", "temperature", "pressure"):
+            assert np.array_equal(host(fs[k]), f[k]), (kind, k)
+        for k in ("rho", "SoS", "Conserved"):
+            e = norm_err(host(fs[k]), f[k])
+            assert max(e) <= TOL, (kind, k, e)
+    say("  kinds 0-2: ghost primitives bit-identical, properties/conserved within 1e-12")

@@ -294,3 +294,52 @@ def test_limited_flux_is_a_positive_convex_blend():
             pred = np.concatenate([rp + lim_f * F[:ns], rq - lim_f * F[:ns]])
             pred_lo = np.concatenate([rp + lim_f * LO[:ns], rq - lim_f * LO[:ns]])
             assert np.all(pred >= np.minimum(1e-60, pred_lo) - 1e-15 * np.max(np.abs(rp)))
+
+
+@pytest.mark.parametrize("side", range(6))
+def test_bc_ghost_values(side):
+    """Ghost cells of one boundary (src/prometeo_bc.rg:1159-1265): the wall
+    tasks take X and p (and T for the adiabatic wall) from the interior
+    neighbour one cell inward along the normal and set u = 0; the isothermal
+    wall takes T from the profile; the Dirichlet task takes X, u, T from the
+    profiles and p from its argument.  The two cells on either side of the
+    ghost layer hold different values, so a neighbour on the wrong side or a
+    swapped field fails."""
+    n = (6, 7, 5)
+    rng = np.random.default_rng(side)
+    st, meta = air_state(8, seed=1)
+    om = oracle_mix(meta)
+    g = O.Grid(*n)
+    f = O.Fields(g, 5)
+    for k, nc in (("MolarFracs", 5), ("velocity", 3), ("MolarFracs_profile", 5), ("velocity_profile", 3)):
+        f[k] = rng.uniform(0.1, 1.0, size=(nc,) + f["rho"].shape)
+    for k in ("pressure", "temperature", "temperature_profile"):
+        f[k] = rng.uniform(0.5, 2.0, size=f["rho"].shape)
+    d, neg = side // 2, side % 2 == 0
+    lo, hi = [0, 0, 0], [n[0] - 1, n[1] - 1, n[2] - 1]
+    lo[d] = hi[d] = 0 if neg else n[d] - 1
+    ax = 2 - d                                   # array axis of direction d in (z, y, x)
+    gi = 0 if neg else n[d] - 1
+    ni = 1 if neg else n[d] - 2
+    take = lambda a, i: np.take(a, i, axis=a.ndim - 3 + ax)
+    before = {k: f[k].copy() for k in f.a}
+    for kind in (0, 1, 2):
+        for k, v in before.items():
+            f[k] = v
+        O.set_bc(g.with_rect(tuple(lo), tuple(hi)), f, kind, side, 0.77, om)
+        if kind == 0:
+            assert np.all(take(f["MolarFracs"], gi) == take(before["MolarFracs_profile"], gi))
+            assert np.all(take(f["velocity"], gi) == take(before["velocity_profile"], gi))
+            assert np.all(take(f["temperature"], gi) == take(before["temperature_profile"], gi))
+            assert np.all(take(f["pressure"], gi) == 0.77)
+        else:
+            assert np.all(take(f["MolarFracs"], gi) == take(before["MolarFracs"], ni))
+            assert np.all(take(f["pressure"], gi) == take(before["pressure"], ni))
+            assert np.all(take(f["velocity"], gi) == 0.0)
+            Tsrc = take(before["temperature"], ni) if kind == 1 else take(before["temperature_profile"], gi)
+            assert np.all(take(f["temperature"], gi) == Tsrc)
+        # nothing outside the ghost layer changes
+        for k in ("MolarFracs", "pressure", "temperature", "velocity"):
+            a, b = f[k], before[k]
+            keep = [i for i in range(n[d]) if i != gi]
+            assert np.all(np.take(a, keep, axis=a.ndim - 3 + ax) == np.take(b, keep, axis=b.ndim - 3 + ax))
