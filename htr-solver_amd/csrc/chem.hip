@@ -521,7 +521,7 @@ chem_group_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
                   htr_view nsteps, int *__restrict__ fails, int *__restrict__ queue, int cell_doubles,
                   int64_t n)
 {
-   extern __shared__ double lds[];
+   HIP_DYNAMIC_SHARED(double, lds)
    using Lay = ChemLayout<MX, PREC>;
    constexpr int N = MX::NEQ;
    const int grp = threadIdx.x/L;

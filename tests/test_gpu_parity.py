@@ -501,9 +501,12 @@ def test_fused_rk_stage_and_device_dt(tgv16, air16p):
     """The fused RK stage (update + primitives + properties in one pass) is
     bit-identical to the three separate tasks, and a full RHS whose deltaTime
     is formed on the device from a radius in device memory equals the host
-    deltaTime call.  Failure flags stay clear on these healthy states."""
+    deltaTime call.  Failure flags stay clear on these healthy states.  The air
+    case leaves the chemistry source out: at 2500-7000 K it is stiff, and an
+    explicit step at the convective CFL limit drives a partial density
+    negative in every cell (the reference advances chemistry implicitly)."""
     import htr_amd as H
-    for name, case, nEq in (("tgv16", tgv16, 5), ("air16 per-seed", air16p, 9)):
+    for name, case, nEq, chem in (("tgv16", tgv16, 5, True), ("air16 per-seed", air16p, 9, False)):
         st, meta, n, om = case
         say("case %s fused RK stage / device dt" % name)
         gm = gpu_mix(meta)
@@ -517,8 +520,8 @@ def test_fused_rk_stage_and_device_dt(tgv16, air16p):
             H.rk_init(fs, nEq)
         for stage, c3 in enumerate((1.0, 0.25, 2.0 / 3.0)):
             RK = [(1.0, 0.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)][stage]
-            H.full_rhs_cfl(a, 0, RK, cfl, c3, radius, True, gm)
-            H.full_rhs(b, 0, RK, dt * c3, True, gm)
+            H.full_rhs_cfl(a, 0, RK, cfl, c3, radius, chem, gm)
+            H.full_rhs(b, 0, RK, dt * c3, chem, gm)
             assert float((a["Conserved_t"] - b["Conserved_t"]).abs().max()) == 0.0
             H.rk_stage(a, stage, gm, cfl=cfl, radius=radius, diag=True)
             H.rk_update(b, nEq, stage, dt)
