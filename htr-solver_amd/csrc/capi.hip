@@ -770,6 +770,27 @@ int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mi
    }
 }
 
+int htr_max_spectral_radius_noalloc(const htr_grid *g, const htr_fields *f, const htr_mix *m,
+                                    double *d_result, void *stream)
+{
+   if (!d_result) return set_error(HTR_E_ARG, "null result");
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo);
+   if (rc < 0) return rc;
+   const int64_t n = rc > 0 ? 0 : exec_cells(geo);   // empty rect: the maximum over no cells, 0
+   if (n > 0) {
+      HTR_NEED(velocity); HTR_NEED(SoS); HTR_NEED(rho); HTR_NEED(mu); HTR_NEED(lam); HTR_NEED(Di);
+      HTR_NEED(temperature); HTR_NEED(MassFracs);
+      for (int d = 0; d < 3; d++) HTR_NEED(m_d[d]);
+   }
+   hipStream_t s = (hipStream_t)stream;
+   switch (kind_of(m)) {
+      case K_CP:  return spectral_radius_noalloc_launch<MixCP>(fr, m->dev, geo, n, d_result, s);
+      case K_AIR: return spectral_radius_noalloc_launch<MixAir>(fr, m->dev, geo, n, d_result, s);
+      case K_30:  return spectral_radius_noalloc_launch<Mix30>(fr, m->dev, geo, n, d_result, s);
+      default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
+   }
+}
+
 int htr_rk_update(const htr_grid *g, const htr_fields *f, int nEq, int stage, double dt, void *stream)
 {
    Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return done(rc);

@@ -59,6 +59,10 @@ template<class MX> int chem_update_launch(const htr_fields &f, const DMix *mix, 
                                           int precision, const htr_view &nsteps, int nreac, hipStream_t s);
 template<class MX> int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g,
                                               double *result, hipStream_t s);
+// the same maximum written to device memory, partials in a file-scope device
+// array (no allocation, no host read-back); n = cells of the rect (0: writes 0)
+template<class MX> int spectral_radius_noalloc_launch(const htr_fields &f, const DMix *mix, const Geo &g, int64_t n,
+                                                      double *d_result, hipStream_t s);
 int chem_set_launch(int block, int blocks_per_cu);
 int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, double dt, hipStream_t s);
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);

@@ -464,6 +464,56 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    return HTR_OK;
 }
 
+// ---- spectral radius without allocation --------------------------------------
+// The partial maxima of htr_max_spectral_radius_noalloc live in a file-scope
+// device array: one double per block of the first pass, which takes at most
+// kRadiusBlocks blocks (grid-stride), so every rect the library accepts fits.
+// Each GPU holds its own copy until the process ends.  In host code the
+// array's name is not a device address: the address on a device comes from
+// hipGetSymbolAddress, looked up once per device with that device current,
+// and only that address reaches the kernels.  Calls that overlap on one GPU
+// share the array and race; every launch rewrites the partials it reads, so
+// nothing needs clearing between calls.
+constexpr int kRadiusBlocks = 4096;
+static __device__ __align__(256) double scratch[kRadiusBlocks];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[64];   // scratch address per device ordinal
+
+static int radius_scratch(double **out)
+{
+   int dev = -1;
+   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(HTR_E_DEVICE, "no HIP device");
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   if (!g_scratch_dev[dev]) {
+      void *p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+         return set_error(HTR_E_DEVICE, "radius scratch: symbol address lookup failed on device %d", dev);
+      g_scratch_dev[dev] = static_cast<double*>(p);
+   }
+   *out = g_scratch_dev[dev];
+   return HTR_OK;
+}
+
+// n = 0 (empty rect): the tail alone writes the maximum over no cells, 0.
+// Launched directly (no timing events: nothing is created on this path).
+template<class MX>
+int spectral_radius_noalloc_launch(const htr_fields &f, const DMix *mix, const Geo &g, int64_t n, double *d_result,
+                                   hipStream_t s)
+{
+   double *partial = nullptr;
+   int rc = radius_scratch(&partial);
+   if (rc) return rc;
+   const int64_t blocks = std::min<int64_t>(grid_for(n, 256), kRadiusBlocks);
+   if (blocks > kRadiusBlocks) return set_error(HTR_E_UNSUPPORTED, "radius scratch holds %d partials", kRadiusBlocks);
+   if (n > 0) {
+      hipLaunchKernelGGL(spectral_radius_kernel<MX>, dim3((unsigned)blocks), dim3(256), 0, s, f, mix, g, n, partial);
+      if ((rc = check_launch("spectral_radius_kernel"))) return rc;
+   }
+   hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(256), 0, s, partial, n > 0 ? (int)blocks : 0, d_result);
+   return check_launch("reduce_max_kernel");
+}
+
 #define HTR_INST_POINT(MX)                                                                          \
    template int properties_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);  \
    template int conserved_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t);   \
@@ -471,6 +521,8 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    template int chem_source_launch<MX>(const htr_fields &, const DMix *, const Geo &, hipStream_t, bool); \
    template int shock_sensor_launch<MX>(int, const htr_fields &, const Geo &, hipStream_t);         \
    template int spectral_radius_launch<MX>(const htr_fields &, const DMix *, const Geo &, double *, hipStream_t); \
+   template int spectral_radius_noalloc_launch<MX>(const htr_fields &, const DMix *, const Geo &, int64_t, double *, \
+                                                   hipStream_t);                                                   \
    template int rk_stage_launch<MX>(const htr_fields &, const DMix *, const Geo &, int, const DtArg &,           \
                                     const htr_view &, hipStream_t);                                               \
    template int bc_launch<MX>(const htr_fields &, const Geo &, int, int, int, double, hipStream_t);

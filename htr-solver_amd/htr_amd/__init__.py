@@ -60,6 +60,7 @@ EXPORTS = [
     "htr_rk_update", "htr_rk_init", "htr_full_rhs",
     "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
     "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl", "htr_set_bc",
+    "htr_max_spectral_radius_noalloc",
 ]
 
 _lib = None
@@ -106,6 +107,7 @@ def _declare(L):
         "htr_chem_source": (i, [G, F, vp, vp]),
         "htr_chem_update": (i, [G, F, d, i, vp, D, vp]),
         "htr_max_spectral_radius": (i, [G, F, vp, dp, vp]),
+        "htr_max_spectral_radius_noalloc": (i, [G, F, vp, vp, vp]),
         "htr_rk_update": (i, [G, F, i, i, d, vp]),
         "htr_rk_init": (i, [G, F, i, vp]),
         "htr_full_rhs": (i, [G, F, i, d, d, d, i, vp, vp]),

@@ -20,6 +20,7 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "update_using_euler_flux", "update_using_hybrid_euler_flux", "update_using_teno_euler_flux",
            "update_using_skew_symmetric_euler_flux", "update_using_diffusion_flux",
            "add_chemistry_sources", "update_chemistry", "calculate_max_spectral_radius",
+           "max_spectral_radius_noalloc",
            "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report",
            "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve", "set_bc",
            "BC_SIDES"]
@@ -286,6 +287,13 @@ def rk_init(fs, nEq, grid=None, stream=None):
 def full_rhs(fs, scheme, RK_coeffs, deltaTime, chem_source, mix, grid=None, stream=None):
     check(lib().htr_full_rhs(_g(fs, grid), C.byref(fs.c()), scheme, RK_coeffs[0], RK_coeffs[1], deltaTime,
                              int(chem_source), mix.h, stream_ptr(stream)))
+
+
+def max_spectral_radius_noalloc(fs, mix, out, grid=None, stream=None):
+    """calculate_max_spectral_radius into the one-element float64 device
+    tensor out, without allocation or host read-back."""
+    check(lib().htr_max_spectral_radius_noalloc(_g(fs, grid), C.byref(fs.c()), mix.h, C.c_void_p(out.data_ptr()),
+                                                stream_ptr(stream)))
 
 
 def full_rhs_cfl(fs, scheme, RK_coeffs, cfl, scale, radius, chem_source, mix, grid=None, stream=None):

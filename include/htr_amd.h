@@ -232,6 +232,20 @@ int htr_chem_launch_config(int lanes_per_cell, int blocks_per_cu);
 int htr_max_spectral_radius(const htr_grid *g, const htr_fields *f, const htr_mix *m,
                             double *result, void *stream);
 
+/* ---- spectral radius without allocation (no reference counterpart) -------- */
+/* For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight stepping or serving loops, harnesses that forbid hipMalloc.
+ * The maximum of htr_max_spectral_radius, written to the device double
+ * *d_result on the stream with no host read-back and no synchronisation (the
+ * device-side ConstantCFL step of htr_full_rhs_cfl / htr_rk_stage reads it
+ * there).  Allocates nothing: the partial maxima go to a file-scope device
+ * array of 4096 doubles (32 KiB, held by every GPU the library runs on until
+ * the process ends); the first pass is grid-stride over at most 4096 blocks,
+ * so every rect the library accepts (fewer than 2^31 cells) fits.  Calls that
+ * overlap on one GPU (two streams at once) share that array and race. */
+int htr_max_spectral_radius_noalloc(const htr_grid *g, const htr_fields *f, const htr_mix *m,
+                                    double *d_result, void *stream);
+
 /* ---- time integration (src/prometeo_rk.rg:48-112) ------------------------- */
 /* SSP-RK3 stage update: Conserved = c1*Conserved_old + c2*Conserved + c3*dt*Conserved_t
  * with (c1,c2,c3) = RK_C[stage] of src/prometeo_const.rg:54-58, stage 0..2 */
