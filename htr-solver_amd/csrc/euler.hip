@@ -1126,7 +1126,8 @@ face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
 static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const htr_view &hv, double RK0,
-                       double RK1, const DtArg &dt, Workspace &w, int64_t nfaces, const htr_view &dec, hipStream_t s)
+                       double RK1, const DtArg &dt, Workspace &w, int64_t nfaces, double *F, const htr_view &dec,
+                       hipStream_t s)
 {
    constexpr int kMaxBlocks = 256*16;
    const TenoBufs b = teno_bufs<MX>(w.teno, nfaces);
@@ -1137,13 +1138,28 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
    int32_t *llist = w.list + nfaces;                  // limiter list after the TENO list
    const unsigned nchunk = (unsigned)std::min<int64_t>((nfaces + 63)/64, 256*8);
    HTR_LAUNCH(HTR_DIR_NAME("euler_teno", DIR), (euler_teno_kernel<MX, DIR, SCHEME, LISTED>), dim3(nchunk),
-              dim3(64*TENO_WAVES), 0, s, f, mix, g, hv, b, RK0, RK1, dt, w.faces, nfaces, w.list, w.counter, dec,
+              dim3(64*TENO_WAVES), 0, s, f, mix, g, hv, b, RK0, RK1, dt, F, nfaces, w.list, w.counter, dec,
               llist, w.counter + 2);
    if ((rc = check_launch("euler_teno_kernel"))) return rc;
    if (MX::NS == 1) return HTR_OK;                   // single species: no positivity limiter
    HTR_LAUNCH(HTR_DIR_NAME("euler_limit", DIR), (euler_limit_kernel<MX, DIR, LISTED>), dim3(256), dim3(256), 0, s,
-              f, mix, g, hv, RK0, RK1, dt, b, w.faces, nfaces, w.list, llist, w.counter + 2, dec);
+              f, mix, g, hv, RK0, RK1, dt, b, F, nfaces, w.list, llist, w.counter + 2, dec);
    return check_launch("euler_limit_kernel");
+}
+
+// The TENO faces of the hybrid sweep along DIR into the face buffer F:
+// classification into the compacted list, Roe records, components and
+// back-projection, limiter blend (the KG faces are left to the caller).
+template<class MX, int DIR>
+static int teno_faces(const htr_fields &f, const DMix *mix, const Geo &g, const htr_view &hv, double RK0, double RK1,
+                      const DtArg &dt, Workspace &w, int64_t nfaces, double *F, const htr_view &dec, hipStream_t s)
+{
+   int rc = zero_counters(w.counter, s);        // TENO list and limiter list counts
+   if (rc) return rc;
+   HTR_LAUNCH(HTR_DIR_NAME("euler_classify", DIR), (euler_classify_kernel<MX, DIR>),
+              dim3((unsigned)ClsTiles<DIR>::blocks(g, nfaces)), dim3(256), 0, s, f, g, nfaces, w.list, w.counter + 1);
+   if ((rc = check_launch("euler_classify_kernel"))) return rc;
+   return launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, F, dec, s);
 }
 
 template<class MX, int DIR, int SCHEME>
@@ -1172,15 +1188,11 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
       HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
       if ((rc = check_launch("enthalpy_kernel"))) return rc;
    }
-   if constexpr (SCHEME != HTR_EULER_KG) {
-      if ((rc = zero_counters(w.counter, s))) return rc;    // TENO list and limiter list counts
+   if constexpr (SCHEME != HTR_EULER_KG && SCHEME != HTR_EULER_HYBRID) {
+      if ((rc = zero_counters(w.counter, s))) return rc;    // limiter list count
    }
    if constexpr (SCHEME == HTR_EULER_HYBRID) {
-      HTR_LAUNCH(HTR_DIR_NAME("euler_classify", DIR), (euler_classify_kernel<MX, DIR>),
-                 dim3((unsigned)ClsTiles<DIR>::blocks(g, nfaces)),
-                 dim3(256), 0, s, f, g, nfaces, w.list, w.counter + 1);
-      if ((rc = check_launch("euler_classify_kernel"))) return rc;
-      if ((rc = launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
+      if ((rc = teno_faces<MX, DIR>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, w.faces, dec, s))) return rc;
    }
    if constexpr (SCHEME == HTR_EULER_HYBRID || SCHEME == HTR_EULER_KG) {
       constexpr bool ALL = SCHEME == HTR_EULER_KG;
@@ -1209,7 +1221,7 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
          return check_launch("euler_kg_walk_kernel");
       }
    } else {
-      if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, dec, s))) return rc;
+      if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, w.faces, dec, s))) return rc;
       HTR_LAUNCH(HTR_DIR_NAME("euler_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)),
                  dim3(256), 0, s, f, g, w.faces, nfaces, f.m_e[DIR], ncells);
       return check_launch("face_divergence_kernel");
@@ -1244,8 +1256,9 @@ template<class MX>
 WorkspaceNeed euler_workspace_need(const Geo &g)
 {
    WorkspaceNeed n;
+   // the full-RHS path keeps the TENO faces of the three directions at once
    auto dir = [&](int64_t nfaces) {
-      n.faces = std::max(n.faces, nfaces*MX::NEQ);
+      n.faces += nfaces*MX::NEQ;
       n.list = std::max(n.list, 2*nfaces);
       n.teno = std::max(n.teno, teno_doubles<MX>(nfaces));
    };

@@ -238,6 +238,7 @@ inline double atomicAdd(double *p, double v)
 template<class T> inline T __builtin_amdgcn_readfirstlane(T v) { return v; }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 inline void __builtin_amdgcn_wave_barrier() { ::hipemu::t_wave->bar.wait(); }
+inline void __builtin_amdgcn_sched_barrier(int) {}
 
 // ---- host API -----------------------------------------------------------------
 inline hipError_t hipMalloc(void **p, size_t n) { *p = malloc(n ? n : 1); return *p ? hipSuccess : hipErrorMemoryAllocation; }

@@ -104,12 +104,35 @@ def main():
             H.update_using_diffusion_flux(fs, d, gpu_mix(meta))
             report("diffusion_d%d" % d, fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
     if "rhs" in which:
+        # the full RHS runs the fused x/y Kennedy-Gruber pass: whole box,
+        # a sub-rect, pure KG, a box with staggered walls in y, and the TGV
         dt = dt_cfl(st, meta, n)
-        g, f = oracle_fields(st, 5, n)
-        O.full_rhs(g, f, O.SCHEME_HYBRID, 1.0, 0.0, dt, 1, om)
-        fs = to_host_fs(st, 5, n)
-        H.full_rhs(fs, 0, (1.0, 0.0), dt, True, gpu_mix(meta))
-        report("full_rhs", fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
+        for label, scheme, lo, hi in (("full_rhs", O.SCHEME_HYBRID, (0, 0, 0), (7, 7, 7)),
+                                      ("full_rhs_subrect", O.SCHEME_HYBRID, (1, 2, 3), (6, 7, 5)),
+                                      ("full_rhs_kg", O.SCHEME_KG, (0, 0, 0), (7, 7, 7))):
+            g, f = oracle_fields(st, 5, n)
+            g = g.with_rect(lo, hi)
+            O.full_rhs(g, f, scheme, 1.0, 0.0, dt, 1, om)
+            fs = to_host_fs(st, 5, n)
+            H.full_rhs(fs, scheme, (1.0, 0.0), dt, True, gpu_mix(meta), grid=fs.grid(lo, hi))
+            report(label, fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
+        stw, metaw, nw = wall_state(8, 6, 8)
+        stw, omw = complete_state(stw, metaw, nw)
+        dtw = dt_cfl(stw, metaw, nw) * 0.5
+        g, f = oracle_fields(stw, 5, nw)
+        g = g.with_rect((0, 1, 0), (7, 6, 7))
+        O.full_rhs(g, f, O.SCHEME_HYBRID, 1.0, 0.0, dtw, 1, omw)
+        fs = to_host_fs(stw, 5, nw)
+        H.full_rhs(fs, 0, (1.0, 0.0), dtw, True, gpu_mix(metaw), grid=fs.grid((0, 1, 0), (7, 6, 7)))
+        report("full_rhs_wall", fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
+        stt, metat = tgv_state(8)
+        nt = (8, 8, 8)
+        stt, omt = complete_state(stt, metat, nt)
+        g, f = oracle_fields(stt, 1, nt)
+        O.full_rhs(g, f, O.SCHEME_HYBRID, 1.0, 0.0, 1e-3, 0, omt)
+        fs = to_host_fs(stt, 1, nt)
+        H.full_rhs(fs, 0, (1.0, 0.0), 1e-3, False, gpu_mix(metat))
+        report("full_rhs_tgv", fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
     if "stage" in which:
         # per-seed composition: a state that stays physical over CFL-0.4
         # steps (the per-cell white-noise composition does not)
