@@ -845,17 +845,10 @@ static int full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double R
    // total enthalpy once for the three sweeps
    rc = (k == K_CP) ? enthalpy_launch<MixCP>(fr, geo, s) : enthalpy_launch<MixAir>(fr, geo, s);
    if (rc) return rc;
-   if (scheme == HTR_EULER_HYBRID || scheme == HTR_EULER_KG) {
-      // the three directions in one Kennedy-Gruber pass per component group
-      rc = (k == K_CP) ? euler3_dispatch<MixCP>(scheme, fr, m->dev, geo, RK0, RK1, dt, s)
-                       : euler3_dispatch<MixAir>(scheme, fr, m->dev, geo, RK0, RK1, dt, s);
+   for (int d = 2; d >= 0; d--) {
+      rc = (k == K_CP) ? euler_dispatch<MixCP>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s, true)
+                       : euler_dispatch<MixAir>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s, true);
       if (rc) return rc;
-   } else {
-      for (int d = 2; d >= 0; d--) {
-         rc = (k == K_CP) ? euler_dispatch<MixCP>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s, true)
-                          : euler_dispatch<MixAir>(d, scheme, fr, m->dev, geo, RK0, RK1, dt, dec, s, true);
-         if (rc) return rc;
-      }
    }
    for (int d = 2; d >= 0; d--) {
       rc = htr_diffusion_flux(d, g, f, m, stream);

@@ -875,34 +875,39 @@ euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, 
    }
 }
 
-// Passes 2b + 2c in one kernel: a block of three waves takes a chunk of 64
-// listed TENO faces at a time.
-//   stage       waves 1 and 2 deposit in LDS, per stencil point of every face
-//               of the chunk, the values the components need (conserved
-//               vector, u_n, p): 6 (nEq + 2) doubles per face;
-//   components  wave w forms components w, w + 3, w + 6, ... of the 64
-//               faces from LDS (the component index is wave-uniform, so every
-//               branch on it is uniform), each wave first forming the
-//               per-point species sums of the left-eigenvector products
-//               (rhs.inl:443-451) once for its components; components and
-//               stencil-selection bits go to LDS;
-//   finish      wave 0 back-projects the 64 faces (add_column, the
-//               reference's summation order), tests the limiter, writes the
-//               face fluxes and decision words and lists the faces that need
-//               the limiter blend,
-// the finish of chunk c overlapping the staging of chunk c + 1.  The Roe
-// record of a face (euler_roe_kernel) is read where it is used, with
-// coalesced loads (list order = record order).  39 KB of LDS per block
-// (air): four blocks, twelve waves per CU.
-constexpr int TENO_WAVES = 3;
+// Passes 2b + 2c in one kernel, pipelined: a block of four waves takes
+// chunks of 64 listed TENO faces.
+//   components  waves 0, 1, 2: wave w forms components w, w + 3, w + 6 of the
+//               64 faces of chunk k from the stencil-point values in LDS (the
+//               component index is wave-uniform, so every branch on it is
+//               uniform): per point the species sums of the left-eigenvector
+//               products (rhs.inl:443-451), the split fluxes on row i, TENO+
+//               and TENO-, -(F+ + F-)/2 and the stencil-selection bits into
+//               LDS (double-buffered by chunk parity);
+//   finish      wave 0, before its components of chunk k: back-projects
+//               chunk k - 1 (add_column, the reference's summation order),
+//               tests the limiter, writes the face fluxes and decision words
+//               and lists the faces that need the limiter blend;
+//   loader      wave 3, meanwhile: issues the gathers of chunk k + 1's
+//               stencil-point values (conserved vector, u_n, p: 6 (nEq + 2)
+//               doubles per face) into its registers;
+//   hand-over   after the component waves are done with chunk k, the loader
+//               writes chunk k + 1's values into LDS.
+// The scattered gathers thus fly while the components of the previous chunk
+// are computed; only the LDS write of the hand-over is on the component
+// waves' path.  The Roe record of a face (euler_roe_kernel) is read by the
+// component waves where it is used, with coalesced loads (list order =
+// record order).  44 KB of LDS per block (air): three blocks, twelve waves
+// per CU.
+constexpr int TENO_WAVES = 4;
 
 template<class MX>
 struct TenoStage {
    static constexpr int NS = MX::NS, NEQ = MX::NEQ;
    static constexpr int NPV = NS + 6;                // per point: rhoY_0..NS-1, rhou 3, rhoE, u_n, p
    double  pt[6*NPV][64];
-   double  fc[NEQ][64];
-   uint8_t bits[NEQ][64];
+   double  fc[2][NEQ][64];
+   uint8_t bits[2][NEQ][64];
 };
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
@@ -918,6 +923,7 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    constexpr int NS = MX::NS, NEQ = MX::NEQ, NPV = St::NPV;
    constexpr int OP = (SCHEME == HTR_EULER_TENOA) ? 1 : (SCHEME == HTR_EULER_TENOLAD) ? 2 : 0;
    constexpr int iN = E_t::iN;
+   constexpr int LOADER = TENO_WAVES - 1;
    __shared__ St S;
    const uint32_t n = (uint32_t)(LISTED ? (int64_t)counter[1] : nfaces);
    const uint32_t nchunks = (n + 63u)/64u;
@@ -925,36 +931,50 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
    const int64_t cap = b.cap;
    E_t E{f, *mix, g, hv};
+   // chunks of this block: blockIdx.x + k gridDim.x, k = 0 .. K - 1
+   const uint32_t K = (blockIdx.x < nchunks) ? (nchunks - 1u - blockIdx.x)/gridDim.x + 1u : 0u;
+   auto chunk = [&](uint32_t k) { return blockIdx.x + k*gridDim.x; };
 
    auto face_of = [&](uint32_t w, P3 &q, int &t) {
       q = Faces<DIR>::cell(g, f, LISTED ? (uint32_t)list[w] : w);
       t = ld<int>(f.nType[DIR], q);
    };
 
-   // waves 1 .. TENO_WAVES-1: point values of chunk c into LDS
-   auto stage = [&](uint32_t c) {
+   // loader: the stencil-point values of the 64 faces of chunk c into pv
+   auto gather = [&](uint32_t c, double *pv) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
-      P3 q = mkp(g, 0, 0, 0);
-      int t = 0;
-      if (valid) face_of(w, q, t);
       P3 P[6];
-      if (valid) E.points(P, q, t);
-      for (int v = wv - 1; v < NPV; v += TENO_WAVES - 1) {
+      if (valid) {
+         P3 q;
+         int t;
+         face_of(w, q, t);
+         E.points(P, q, t);
+      }
+      if (!valid)
 #pragma unroll
-         for (int k = 0; k < 6; k++) {
-            double x = 0.0;
-            if (valid)
-               x = (v < NS + 4) ? ld<double>(f.Conserved, P[k], v) : (v == NS + 4) ? ld<double>(f.velocity, P[k], iN)
-                                                                      : ld<double>(f.pressure, P[k]);
-            S.pt[k*NPV + v][lane] = x;
-         }
+         for (int k = 0; k < 6; k++) P[k] = mkp(g, 0, 0, 0);      // a valid point; the values are not used
+      // one point at a time: the addresses of the next point's loads are
+      // formed after this point's loads are issued (hoisted, they would hold
+      // some 100 VGPRs next to the 6 (nEq + 2) values)
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+         __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+         for (int v = 0; v < NPV; v++)
+            pv[k*NPV + v] = (v < NS + 4) ? ld<double>(f.Conserved, P[k], v)
+                          : (v == NS + 4) ? ld<double>(f.velocity, P[k], iN) : ld<double>(f.pressure, P[k]);
       }
    };
+   auto hand_over = [&](const double *pv) {
+#pragma unroll
+      for (int k = 0; k < 6*NPV; k++) S.pt[k][lane] = pv[k];
+   };
 
-   // wave 0: back-projection, limiter test, face flux and decision word of
-   // the 64 faces of chunk c (rhs.inl:708-721)
-   auto finish = [&](uint32_t c) {
+   // loader: back-projection, limiter test, face flux and decision word of
+   // the 64 faces of chunk c, whose components are in buffer bb
+   // (rhs.inl:708-721)
+   auto finish = [&](uint32_t c, int bb) {
       const uint32_t w = c*64u + (uint32_t)lane;
       bool limit = false;
       if (w < n) {
@@ -970,9 +990,9 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
          uint32_t h = 0;
 #pragma unroll
          for (int i = 0; i < NEQ; i++) {
-            E_t::add_column(Flux, sumQsp, i, S.fc[i][lane], R, ERoe, dedp);
-            const uint32_t bb = S.bits[i][lane];
-            h = fold_bits(fold_bits(h, bb & 15u), bb >> 4);
+            E_t::add_column(Flux, sumQsp, i, S.fc[bb][i][lane], R, ERoe, dedp);
+            const uint32_t bt = S.bits[bb][i][lane];
+            h = fold_bits(fold_bits(h, bt & 15u), bt >> 4);
          }
          if (NS > 1) {
             const P3 pP1 = wplus(g, DIR, q, c_st.Cp[t][2]);
@@ -999,85 +1019,107 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
       }
    };
 
-   uint32_t last = 0;
-   bool any = false;
-   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-      if (wv > 0) stage(c);
-      else if (any) finish(last);
-      __syncthreads();
-      // ---- components wv, wv + TENO_WAVES, ... of the 64 faces
+   // component waves: components wv, wv + 3, wv + 6 of chunk c into buffer bb
+   auto components = [&](uint32_t c, int bb) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
       int tt = 0;
       if (valid) { P3 q; face_of(w, q, tt); }
       const bool all_std = __ballot(tt != Std_node) == 0ull;
-      if (valid) {
-         typename E_t::Roe R;
+      if (!valid) return;
+      typename E_t::Roe R;
 #pragma unroll
-         for (int j = 0; j < 3; j++) R.v[j] = b.rec[(int64_t)(NS + j)*cap + w];
-         R.a = b.rec[(int64_t)(NS + 3)*cap + w];
-         R.e = b.rec[(int64_t)(NS + 5)*cap + w];
-         R.rho = b.rec[(int64_t)(NS + 6)*cap + w];
-         R.dpde = b.rec[(int64_t)(NS + 7)*cap + w];
-         R.a2 = b.rec[(int64_t)(2*NS + 8)*cap + w];
-         typename E_t::Proj J;
-         E_t::proj(J, R);
-         // weights of the species sums of the left-eigenvector products
-         // (rhs.inl:443-451)
-         double wj[NS];
+      for (int j = 0; j < 3; j++) R.v[j] = b.rec[(int64_t)(NS + j)*cap + w];
+      R.a = b.rec[(int64_t)(NS + 3)*cap + w];
+      R.e = b.rec[(int64_t)(NS + 5)*cap + w];
+      R.rho = b.rec[(int64_t)(NS + 6)*cap + w];
+      R.dpde = b.rec[(int64_t)(NS + 7)*cap + w];
+      R.a2 = b.rec[(int64_t)(2*NS + 8)*cap + w];
+      typename E_t::Proj J;
+      E_t::proj(J, R);
+      // weights of the species sums of the left-eigenvector products
+      // (rhs.inl:443-451)
+      double wj[NS];
 #pragma unroll
-         for (int j = 0; j < NS; j++) wj[j] = (J.Coeff - b.rec[(int64_t)(NS + 8 + j)*cap + w])*J.iaRoe2;
-         for (int i = wv; i < NEQ; i += TENO_WAVES) {
-            // the point values are re-read from LDS and the species sums
-            // re-formed for every component: held in registers across the
-            // loop they would cost some 120 VGPRs
-            asm volatile("" ::: "memory");
-            const double lam = b.rec[(int64_t)(2*NS + ((i == 0) ? 9 : (i == NEQ-1) ? 11 : 10))*cap + w];
-            const double Ysel = (i >= 1 && i <= NS) ? b.rec[(int64_t)(i - 1)*cap + w] : 0.0;
-            double vp[6], vm[6];
+      for (int j = 0; j < NS; j++) wj[j] = (J.Coeff - b.rec[(int64_t)(NS + 8 + j)*cap + w])*J.iaRoe2;
+      for (int i = wv; i < NEQ; i += LOADER) {
+         // the point values are re-read from LDS and the species sums
+         // re-formed for every component: held in registers across the
+         // loop they would cost some 120 VGPRs
+         asm volatile("" ::: "memory");
+         const double lam = b.rec[(int64_t)(2*NS + ((i == 0) ? 9 : (i == NEQ-1) ? 11 : 10))*cap + w];
+         const double Ysel = (i >= 1 && i <= NS) ? b.rec[(int64_t)(i - 1)*cap + w] : 0.0;
+         double vp[6], vm[6];
 #pragma unroll
-            for (int k = 0; k < 6; k++) {
-               const double un = S.pt[k*NPV + NS + 4][lane];
-               const double pr = S.pt[k*NPV + NS + 5][lane];
-               double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0, qs = 0.0;
+         for (int k = 0; k < 6; k++) {
+            const double un = S.pt[k*NPV + NS + 4][lane];
+            const double pr = S.pt[k*NPV + NS + 5][lane];
+            double s1q = 0.0, sqq = 0.0, s1f = 0.0, sqf = 0.0, qs = 0.0;
 #pragma unroll
-               for (int j = 0; j < NS; j++) {
-                  const double qv = S.pt[k*NPV + j][lane];
-                  if (j == i - 1) qs = qv;
-                  s1q += wj[j]*qv;
-                  sqq += qv;
-                  const double fq = qv*un;
-                  s1f += wj[j]*fq;
-                  sqf += fq;
-               }
-               double qm[4], fm[4];
-#pragma unroll
-               for (int j = 0; j < 4; j++) { qm[j] = S.pt[k*NPV + NS + j][lane]; fm[j] = qm[j]*un; }
-               fm[iN] += pr;
-               fm[3] += pr*un;
-               const double Q = E_t::char_row(i, R, J, s1q, sqq, qs, qm, Ysel);
-               const double Fv = E_t::char_row(i, R, J, s1f, sqf, qs*un, fm, Ysel);
-               vp[k] = Fv + lam*Q;
-               vm[k] = Fv - lam*Q;
+            for (int j = 0; j < NS; j++) {
+               const double qv = S.pt[k*NPV + j][lane];
+               if (j == i - 1) qs = qv;
+               s1q += wj[j]*qv;
+               sqq += qv;
+               const double fq = qv*un;
+               s1f += wj[j]*fq;
+               sqf += fq;
             }
-            uint32_t bp, bm;
-            double FPlus, FMinus;
-            if (OP != 1 && all_std) {
-               FPlus  = teno_reconstruct<OP, 0, true>(vp, tt, &bp);
-               FMinus = teno_reconstruct<OP, 1, true>(vm, tt, &bm);
-            } else {
-               FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
-               FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
-            }
-            S.fc[i][lane] = -0.5*(FPlus + FMinus);
-            S.bits[i][lane] = (uint8_t)(bp | (bm << 4));
+            double qm[4], fm[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) { qm[j] = S.pt[k*NPV + NS + j][lane]; fm[j] = qm[j]*un; }
+            fm[iN] += pr;
+            fm[3] += pr*un;
+            const double Q = E_t::char_row(i, R, J, s1q, sqq, qs, qm, Ysel);
+            const double Fv = E_t::char_row(i, R, J, s1f, sqf, qs*un, fm, Ysel);
+            vp[k] = Fv + lam*Q;
+            vm[k] = Fv - lam*Q;
          }
+         uint32_t bp, bm;
+         double FPlus, FMinus;
+         if (OP != 1 && all_std) {
+            FPlus  = teno_reconstruct<OP, 0, true>(vp, tt, &bp);
+            FMinus = teno_reconstruct<OP, 1, true>(vm, tt, &bm);
+         } else {
+            FPlus  = teno_reconstruct<OP, 0>(vp, tt, &bp);
+            FMinus = teno_reconstruct<OP, 1>(vm, tt, &bm);
+         }
+         S.fc[bb][i][lane] = -0.5*(FPlus + FMinus);
+         S.bits[bb][i][lane] = (uint8_t)(bp | (bm << 4));
+      }
+   };
+
+   // The two roles run their own loops (wave-uniform branch), so that the
+   // loader's gathered values are not live across the component code; both
+   // loops reach the same 1 + 2K barriers.  Wave 0 also back-projects the
+   // previous chunk before its components (held next to the gathered values
+   // in the loader, the back-projection's registers would double its
+   // footprint).
+   if (K == 0) return;                          // uniform over the block
+   if (wv == LOADER) {
+      {
+         double pv[6*NPV];
+         gather(chunk(0), pv);
+         hand_over(pv);
       }
       __syncthreads();
-      last = c;
-      any = true;
+      for (uint32_t k = 0; k < K; k++) {
+         double pv[6*NPV];                      // live from the gather to the hand-over only
+         if (k + 1 < K) gather(chunk(k + 1), pv);
+         __syncthreads();                       // chunk k's components are in LDS, its point values are free
+         if (k + 1 < K) hand_over(pv);
+         __syncthreads();                       // chunk k + 1's point values are in LDS
+      }
+   } else {
+      __syncthreads();
+      for (uint32_t k = 0; k < K; k++) {
+         if (wv == 0 && k > 0) finish(chunk(k - 1), (int)((k - 1u) & 1u));
+         components(chunk(k), (int)(k & 1u));
+         __syncthreads();
+         __syncthreads();
+      }
+      if (wv == 0) finish(chunk(K - 1), (int)((K - 1u) & 1u));
    }
-   if (wv == 0 && any) finish(last);
 }
 
 template<class MX, int DIR, bool LISTED>
@@ -1124,6 +1166,21 @@ face_divergence_kernel(const htr_fields f, const Geo g, const double *__restrict
       at<double>(f.Conserved_t, p, c) += (face_ld<NEQ>(F, fp, c) - face_ld<NEQ>(F, fm, c))*m;
 }
 
+// Blocks of the TENO component kernel: persistent, three per CU (what the LDS
+// and registers of one block allow), each looping over chunks.
+// HTR_TENO_BLOCKS=n caps the grid at n blocks (tuning; a small cap makes
+// every block take many chunks).
+static int teno_blocks()
+{
+   static int cap = -1;
+   if (cap < 0) {
+      int v = 256*3;
+      if (const char *e = getenv("HTR_TENO_BLOCKS")) { const int x = atoi(e); if (x > 0) v = x; }
+      cap = v;
+   }
+   return cap;
+}
+
 template<class MX, int DIR, int SCHEME, bool LISTED>
 static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const htr_view &hv, double RK0,
                        double RK1, const DtArg &dt, Workspace &w, int64_t nfaces, double *F, const htr_view &dec,
@@ -1136,7 +1193,7 @@ static int launch_teno(const htr_fields &f, const DMix *mix, const Geo &g, const
                       f, mix, g, hv, b, nfaces, w.list, w.counter);
    int rc = check_launch("euler_roe_kernel"); if (rc) return rc;
    int32_t *llist = w.list + nfaces;                  // limiter list after the TENO list
-   const unsigned nchunk = (unsigned)std::min<int64_t>((nfaces + 63)/64, 256*8);
+   const unsigned nchunk = (unsigned)std::min<int64_t>((nfaces + 63)/64, teno_blocks());
    HTR_LAUNCH(HTR_DIR_NAME("euler_teno", DIR), (euler_teno_kernel<MX, DIR, SCHEME, LISTED>), dim3(nchunk),
               dim3(64*TENO_WAVES), 0, s, f, mix, g, hv, b, RK0, RK1, dt, F, nfaces, w.list, w.counter, dec,
               llist, w.counter + 2);
@@ -1256,9 +1313,8 @@ template<class MX>
 WorkspaceNeed euler_workspace_need(const Geo &g)
 {
    WorkspaceNeed n;
-   // the full-RHS path keeps the TENO faces of the three directions at once
    auto dir = [&](int64_t nfaces) {
-      n.faces += nfaces*MX::NEQ;
+      n.faces = std::max(n.faces, nfaces*MX::NEQ);
       n.list = std::max(n.list, 2*nfaces);
       n.teno = std::max(n.teno, teno_doubles<MX>(nfaces));
    };

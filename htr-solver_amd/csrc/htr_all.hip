@@ -2,7 +2,6 @@
 // __constant__ stencil tables are one object without relocatable device code.
 #include "tables.hip"
 #include "euler.hip"
-#include "euler3.hip"
 #include "diffusion.hip"
 #include "pointwise.hip"
 #include "chem.hip"

@@ -46,12 +46,6 @@ template<class MX> int enthalpy_launch(const htr_fields &f, const Geo &g, hipStr
 template<class MX> int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
                                       double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s,
                                       bool enthalpy_ready = false);
-// hybrid or pure KG Euler fluxes of the three directions in the reference's
-// order (Z, Y, X) for the full RHS: TENO passes per direction, then one
-// fused Kennedy-Gruber pass per component group (the enthalpy array must be
-// formed on the stream first)
-template<class MX> int euler3_dispatch(int scheme, const htr_fields &f, const DMix *mix, const Geo &g, double RK0,
-                                       double RK1, const DtArg &dt, hipStream_t s);
 template<class MX> int diffusion_dispatch(int dir, const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int properties_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int conserved_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);

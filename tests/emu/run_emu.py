@@ -79,6 +79,15 @@ def main():
         euler(st, meta, n, om, 0, O.SCHEME_KG, "kg")
         # a sub-rect that starts inside the box (segment and row offsets)
         euler(st, meta, n, om, 1, O.SCHEME_HYBRID, "subrect", lo=(1, 2, 3), hi=(6, 7, 5))
+    if "pipeline" in which:
+        # few TENO blocks (HTR_TENO_BLOCKS, set by the caller): every block
+        # takes several chunks, so the loader's gather of chunk k + 1 overlaps
+        # the components of chunk k
+        st3, meta3 = air_state(12, seed=6)
+        n3 = (12, 12, 12)
+        st3, om3 = complete_state(st3, meta3, n3)
+        for d in range(3):
+            euler(st3, meta3, n3, om3, d, O.SCHEME_TENO, "pipeline")
     if "limiter" in which:
         st2 = dict(st)
         # a nearly vanishing species next to a rich one: the positivity

@@ -37,10 +37,10 @@ def emu_lib(tmp_path_factory):
     return out
 
 
-def run_cases(lib, *which):
+def run_cases(lib, *which, env_extra=None):
     asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
     env = dict(os.environ, HTR_AMD_LIBRARY=lib, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0",
-               UBSAN_OPTIONS="print_stacktrace=1")
+               UBSAN_OPTIONS="print_stacktrace=1", **(env_extra or {}))
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_emu.py"), *which], env=env,
                        capture_output=True, text=True, timeout=900)
     cases = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("CASE ")]
@@ -52,6 +52,15 @@ def run_cases(lib, *which):
 def test_euler_sweeps_in_bounds_and_bit_identical(emu_lib):
     res = run_cases(emu_lib, "euler", "limiter", "wall")
     assert len(res) >= 20
+    assert all(v == 0.0 for v in res.values()), res
+
+
+def test_teno_pipeline_over_many_chunks(emu_lib):
+    """Three TENO blocks for ~27 chunks per sweep: the gather of the next
+    chunk, the hand-over and the double-buffered components of every block
+    are exercised, bit-identical to the oracle."""
+    res = run_cases(emu_lib, "pipeline", env_extra={"HTR_TENO_BLOCKS": "3"})
+    assert len(res) == 6
     assert all(v == 0.0 for v in res.values()), res
 
 
