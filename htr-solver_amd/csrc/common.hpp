@@ -90,6 +90,18 @@ __device__ inline P3 wminus(const Geo &g, int d, P3 p, int off) { return wplus(g
 
 inline int64_t exec_cells(const Geo &g) { return (int64_t)g.esz[0]*g.esz[1]*g.esz[2]; }
 
+// XCD-aware block order: the dispatcher hands workgroups to the eight XCDs
+// round robin (block b runs on XCD b % 8), and each XCD has its own L2.  The
+// logical index makes blocks with consecutive logical indices share an XCD
+// (XCD x holds logical blocks [x n/8, (x+1) n/8)), so work items that touch
+// the same cache lines -- neighbouring chunks of a face list -- meet in one
+// L2.  Identity when n is not a multiple of 8.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t n)
+{
+   if (n & 7u) return b;
+   return (b & 7u)*(n >> 3) + (b >> 3);
+}
+
 // point of the execution rect for a flat index, x fastest (the host keeps
 // the cell count below 2^31)
 __device__ inline P3 exec_point(const Geo &g, int64_t i64)

@@ -931,9 +931,12 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
    const int64_t cap = b.cap;
    E_t E{f, *mix, g, hv};
-   // chunks of this block: blockIdx.x + k gridDim.x, k = 0 .. K - 1
-   const uint32_t K = (blockIdx.x < nchunks) ? (nchunks - 1u - blockIdx.x)/gridDim.x + 1u : 0u;
-   auto chunk = [&](uint32_t k) { return blockIdx.x + k*gridDim.x; };
+   // chunks of this block: L + k gridDim.x, k = 0 .. K - 1, with L the
+   // XCD-aware logical block index (the chunks in flight on one XCD are
+   // neighbours in the list and share stencil cache lines)
+   const uint32_t L = xcd_block(blockIdx.x, gridDim.x);
+   const uint32_t K = (L < nchunks) ? (nchunks - 1u - L)/gridDim.x + 1u : 0u;
+   auto chunk = [&](uint32_t k) { return L + k*gridDim.x; };
 
    auto face_of = [&](uint32_t w, P3 &q, int &t) {
       q = Faces<DIR>::cell(g, f, LISTED ? (uint32_t)list[w] : w);
