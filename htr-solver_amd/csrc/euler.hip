@@ -444,12 +444,16 @@ __device__ inline void store_face(double *__restrict__ F, int64_t nfaces, uint32
 // along the row, 4096 consecutive face ids.
 constexpr int CLS_PER_THREAD = 16;
 
-template<int DIR>
+template<int DIR, int TX = 16, int TJ = 32>
 struct ClsTiles {
    static constexpr int B = Faces<DIR>::B;
-   __host__ __device__ static uint32_t ntx(const Geo &g) { return ((uint32_t)g.esz[0] + 63u)/64u; }
-   __host__ __device__ static uint32_t ntj(const Geo &g) { return ((uint32_t)g.esz[DIR] + 8u)/8u; }
-   __host__ __device__ static uint32_t ntb(const Geo &g) { return ((uint32_t)g.esz[B] + 7u)/8u; }
+   static constexpr int TB = 4096/(TX*TJ);
+   static_assert(TX*TJ*TB == 4096 && (TX & (TX - 1)) == 0 && (TJ & (TJ - 1)) == 0, "tile of 4096 faces");
+   static constexpr int LX = (TX == 64) ? 6 : (TX == 32) ? 5 : (TX == 16) ? 4 : 3;
+   static constexpr int LJ = (TJ == 64) ? 6 : (TJ == 32) ? 5 : (TJ == 16) ? 4 : (TJ == 8) ? 3 : 2;
+   __host__ __device__ static uint32_t ntx(const Geo &g) { return ((uint32_t)g.esz[0] + TX - 1u)/TX; }
+   __host__ __device__ static uint32_t ntj(const Geo &g) { return ((uint32_t)g.esz[DIR] + TJ)/TJ; }
+   __host__ __device__ static uint32_t ntb(const Geo &g) { return ((uint32_t)g.esz[B] + TB - 1u)/TB; }
    static int64_t blocks(const Geo &g, int64_t nfaces)
    {
       if (DIR == 0) return (nfaces + 4095)/4096;
@@ -465,7 +469,7 @@ struct ClsTiles {
       }
       const uint32_t nx = ntx(g), nj = ntj(g);
       const uint32_t tx = b % nx, r = b/nx, tj = r % nj, tb = r/nj;
-      const uint32_t x = tx*64u + (e & 63u), j = tj*8u + ((e >> 6) & 7u), c = tb*8u + (e >> 9);
+      const uint32_t x = tx*TX + (e & (TX - 1u)), j = tj*TJ + ((e >> LX) & (TJ - 1u)), c = tb*TB + (e >> (LX + LJ));
       if (x >= (uint32_t)g.esz[0] || j > (uint32_t)g.esz[DIR] || c >= (uint32_t)g.esz[B]) return false;
       fid = j*((uint32_t)g.esz[0]*(uint32_t)g.esz[B]) + x + c*(uint32_t)g.esz[0];
       return true;

@@ -315,7 +315,7 @@ def isentropic_vortex(n, width=10.0, beta=5.0, gamma=1.4):
 
 
 # ---- wall-bounded direction -----------------------------------------------------
-STD, L_S, LP1_S, LP2_S, RM3_S, RM2_S, RM1_S, R_S = range(8)
+STD, L_S, LP1_S, LP2_S, RM3_S, RM2_S, RM1_S, R_S, L_C, LP1_C, RM2_C, RM1_C, R_C = range(13)
 
 
 def tanh_minus_coords(n, width, stretching):
@@ -327,14 +327,20 @@ def tanh_minus_coords(n, width, stretching):
     return np.concatenate([[0.0], x, [width]])
 
 
-def wall_metrics(n_interior, width=1.0, stretching=1.0):
-    """Node types and metrics of a direction with staggered walls on both
-    ends: returns (nType, m_e, m_d, m_s, centres) of length n_interior + 2."""
+def wall_metrics(n_interior, width=1.0, stretching=1.0, collocated=False):
+    """Node types and metrics of a direction with walls on both ends,
+    staggered (L_S .. R_S, src/prometeo_metric_coeffs.h:55-67) or collocated
+    (L_C, Lp1_C .. Rm2_C, Rm1_C, R_C): returns (nType, m_e, m_d, m_s,
+    centres) of length n_interior + 2."""
     x = tanh_minus_coords(n_interior, width, stretching)
     n = len(x)
     nt = np.full(n, STD, np.int32)
-    nt[0], nt[1], nt[2] = L_S, LP1_S, LP2_S
-    nt[-1], nt[-2], nt[-3], nt[-4] = R_S, RM1_S, RM2_S, RM3_S
+    if collocated:
+        nt[0], nt[1] = L_C, LP1_C
+        nt[-1], nt[-2], nt[-3] = R_C, RM1_C, RM2_C
+    else:
+        nt[0], nt[1], nt[2] = L_S, LP1_S, LP2_S
+        nt[-1], nt[-2], nt[-3], nt[-4] = R_S, RM1_S, RM2_S, RM3_S
     # face coordinates: midpoint between neighbouring centres (ghosts sit on faces)
     xf = np.empty(n + 1)
     xf[1:-1] = 0.5 * (x[:-1] + x[1:])
@@ -348,7 +354,8 @@ def wall_metrics(n_interior, width=1.0, stretching=1.0):
         me[i] = 1.0 / (hi - lo) if hi > lo else 1.0 / (x[min(i + 1, n - 1)] - x[max(i - 1, 0)])
     md = np.empty(n)
     ms = np.empty(n)
-    grad = {STD: (0.5, 0.5), L_S: (0.0, 2.0), LP1_S: (1.0, 0.5), RM1_S: (0.5, 1.0), R_S: (2.0, 0.0)}
+    grad = {STD: (0.5, 0.5), L_S: (0.0, 2.0), LP1_S: (1.0, 0.5), RM1_S: (0.5, 1.0), R_S: (2.0, 0.0),
+            L_C: (0.0, 1.0), R_C: (1.0, 0.0)}
     for i in range(n):
         g0, g1 = grad.get(int(nt[i]), (0.5, 0.5))
         xm, xp = x[max(i - 1, 0)], x[min(i + 1, n - 1)]

@@ -169,11 +169,22 @@ def main():
             if fl != 0:
                 sys.exit(5)
     if "wall" in which:
-        stw, metaw, nw = wall_state(8, 6, 8)
-        stw, omw = complete_state(stw, metaw, nw)
-        dtw = dt_cfl(stw, metaw, nw) * 0.5
-        for scheme, label in ((O.SCHEME_HYBRID, "wall_hybrid"), (O.SCHEME_TENO, "wall_teno")):
-            euler(stw, metaw, nw, omw, 1, scheme, label, dt=dtw, lo=(0, 1, 0), hi=(7, 6, 7))
+        for colloc in (False, True):
+            stw, metaw, nw = wall_state(8, 6, 8, collocated=colloc)
+            stw, omw = complete_state(stw, metaw, nw)
+            dtw = dt_cfl(stw, metaw, nw) * 0.5
+            tag = "colloc" if colloc else "wall"
+            for scheme, label in ((O.SCHEME_HYBRID, "hybrid"), (O.SCHEME_TENO, "teno"), (O.SCHEME_KG, "kg")):
+                euler(stw, metaw, nw, omw, 1, scheme, tag + "_" + label, dt=dtw, lo=(0, 1, 0), hi=(7, 6, 7))
+            for d in range(3):
+                g, f = oracle_fields(stw, 5, nw)
+                g = g.with_rect((0, 1, 0), (7, 6, 7))
+                f["Conserved_t"] = 0.0
+                O.diffusion_flux(g, f, d, omw)
+                fs = to_host_fs(stw, 5, nw)
+                fs["Conserved_t"] = 0.0
+                H.update_using_diffusion_flux(fs, d, gpu_mix(metaw), grid=fs.grid((0, 1, 0), (7, 6, 7)))
+                report("%s_diffusion_d%d" % (tag, d), fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
     if "bc" in which:
         prof = {"MolarFracs_profile": st["MolarFracs"][:, ::-1].copy(),
                 "velocity_profile": 0.1 * st["velocity"][:, ::-1].copy(),

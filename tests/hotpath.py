@@ -117,9 +117,9 @@ def tgv_state(n):
     return cases.tgv(n)
 
 
-def wall_state(nx, ny, nz, seed=3):
-    """Air box with y bounded by staggered walls (node types L_S .. R_S) on a
-    stretched grid; x and z periodic."""
+def wall_state(nx, ny, nz, seed=3, collocated=False):
+    """Air box with y bounded by walls on a stretched grid (staggered node
+    types L_S .. R_S, or collocated L_C .. R_C); x and z periodic."""
     st, meta = cases.air_box(max(nx, nz), seed)
     rng = np.random.default_rng(seed)
     n = (nx, ny + 2, nz)
@@ -132,7 +132,7 @@ def wall_state(nx, ny, nz, seed=3):
         a = np.concatenate([a] * reps, axis=-2)[..., : ny + 2, :]
         out[k] = np.ascontiguousarray(a)
     # boundary layer-like profile in y
-    nt, me, md, ms, y = cases.wall_metrics(ny, width=1.0, stretching=1.5)
+    nt, me, md, ms, y = cases.wall_metrics(ny, width=1.0, stretching=1.5, collocated=collocated)
     prof = np.tanh(4 * y / y[-1])
     out["velocity"] = out["velocity"] * prof[None, None, :, None]
     px = cases.periodic_metrics(nx)

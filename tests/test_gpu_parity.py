@@ -456,13 +456,15 @@ def test_concurrent_streams():
         assert d == 0.0
 
 
-def test_wall_bounded_node_types(parity_record):
-    """y bounded by staggered walls on a stretched grid: every staggered node
-    type (L_S, Lp1_S, Lp2_S, Rm3_S, Rm2_S, Rm1_S, R_S) on the Euler and
-    diffusion paths, execution rect = interior cells in y."""
+@pytest.mark.parametrize("collocated", [False, True], ids=["staggered", "collocated"])
+def test_wall_bounded_node_types(parity_record, collocated):
+    """y bounded by walls on a stretched grid: every staggered node type
+    (L_S, Lp1_S, Lp2_S, Rm3_S, Rm2_S, Rm1_S, R_S), or every collocated one
+    (L_C, Lp1_C, Rm2_C, Rm1_C, R_C), on the Euler and diffusion paths,
+    execution rect = interior cells in y."""
     import htr_amd as H
-    st, meta, n = wall_state(12, 14, 8)
-    say("case wall 12x16x8 node types")
+    st, meta, n = wall_state(12, 14, 8, collocated=collocated)
+    say("case wall 12x16x8 node types (%s)" % ("collocated" if collocated else "staggered"))
     st, om = complete_state(st, meta, n)
     dt = dt_cfl(st, meta, n) * 0.3
     hi = (n[0] - 1, n[1] - 2, n[2] - 1)
@@ -473,7 +475,8 @@ def test_wall_bounded_node_types(parity_record):
     for scheme, label in ((O.SCHEME_HYBRID, "hybrid"), (O.SCHEME_TENO, "TENO"), (O.SCHEME_KG, "KG")):
         ref, gpu, do, dg = euler_case(st, meta, n, om, 1, scheme, RK=(1.0, 0.0), dt=dt,
                                       exec_lo=(0, 1, 0), exec_hi=hi)
-        check_euler(ref, gpu, do, dg, "wall " + label, 1, faces=faces, record=parity_record)
+        check_euler(ref, gpu, do, dg, ("colloc " if collocated else "wall ") + label, 1, faces=faces,
+                    record=parity_record)
     for d in range(3):
         g, f = oracle_fields(st, 5, n)
         g = g.with_rect((0, 1, 0), hi)
