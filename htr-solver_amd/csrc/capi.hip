@@ -383,7 +383,8 @@ static int layout_views(const htr_grid *g, htr_fields &out, htr_view *extra, int
       {&out.shockSensor[0], 1, "shockSensor[0]"}, {&out.shockSensor[1], 1, "shockSensor[1]"},
       {&out.shockSensor[2], 1, "shockSensor[2]"},
       {&out.MolarFracs_profile, 8, "MolarFracs_profile"}, {&out.velocity_profile, 8, "velocity_profile"},
-      {&out.temperature_profile, 8, "temperature_profile"} };
+      {&out.temperature_profile, 8, "temperature_profile"},
+      {&out.dudtBoundary, 8, "dudtBoundary"}, {&out.dTdtBoundary, 8, "dTdtBoundary"} };
    for (const Named &n : all) {
       const int rc = L.add(*n.v, n.esize, n.name, blo);
       if (rc) return rc;
@@ -604,6 +605,15 @@ void htr_mix_destroy(htr_mix *m)
    delete m;
 }
 
+// every mixture instance (the point-wise and boundary tasks)
+#define HTR_DISPATCH_ALL(call, ...)                                      \
+   switch (kind_of(m)) {                                                 \
+      case K_CP:  return call<MixCP>(__VA_ARGS__);                       \
+      case K_AIR: return call<MixAir>(__VA_ARGS__);                      \
+      case K_30:  return call<Mix30>(__VA_ARGS__);                       \
+      default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec); \
+   }
+
 #define HTR_DISPATCH3(call, ...)                                         \
    switch (kind_of(m)) {                                                 \
       case K_CP:  return call<MixCP>(__VA_ARGS__);                       \
@@ -704,6 +714,49 @@ int htr_set_bc(int kind, int side, const htr_grid *g, const htr_fields *f, doubl
       case K_30:  return bc_launch<Mix30>(fr, geo, kind, dir, inward, Pbc, s);
       default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
    }
+}
+
+// the interior points of a boundary rect (one cell inward along the normal)
+// lie inside the bounds
+static int nscbc_rect(int side, const htr_grid *g)
+{
+   if (side < 0 || side > 5) return set_error(HTR_E_ARG, "bad NSCBC side %d", side);
+   const int dir = side/2;
+   if (side % 2 == 0 ? (g->exec.hi[dir] + 1 > g->bounds.hi[dir]) : (g->exec.lo[dir] - 1 < g->bounds.lo[dir]))
+      return set_error(HTR_E_ARG, "NSCBC: the interior points of the boundary rect are outside the bounds");
+   return HTR_OK;
+}
+
+int htr_nscbc_inflow(int side, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)
+{
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
+   if ((rc = nscbc_rect(side, g))) return rc;
+   const int dir = side/2;
+   HTR_NEED(MassFracs); HTR_NEED(pressure); HTR_NEED(velocity); HTR_NEED(rho); HTR_NEED(SoS);
+   HTR_NEED(temperature); HTR_NEED(dudtBoundary); HTR_NEED(dTdtBoundary); HTR_NEED(Conserved_t);
+   HTR_NEED(nType[dir]); HTR_NEED(m_d[dir]);
+   HTR_DISPATCH_ALL(nscbc_inflow_launch, dir, side % 2 == 0, fr, m->dev, geo, (hipStream_t)stream);
+}
+
+int htr_nscbc_outflow(int side, const htr_grid *g, const htr_fields *f, double MaxMach, double LengthScale,
+                      double PInf, const htr_mix *m, void *stream)
+{
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
+   if ((rc = nscbc_rect(side, g))) return rc;
+   HTR_NEED(MassFracs); HTR_NEED(pressure); HTR_NEED(velocity); HTR_NEED(rho); HTR_NEED(mu); HTR_NEED(SoS);
+   HTR_NEED(temperature); HTR_NEED(Conserved); HTR_NEED(Conserved_t);
+   for (int d = 0; d < 3; d++) { HTR_NEED(nType[d]); HTR_NEED(m_d[d]); }
+   HTR_DISPATCH_ALL(nscbc_outflow_launch, side/2, side % 2 == 0, fr, m->dev, geo, MaxMach, LengthScale, PInf,
+                    (hipStream_t)stream);
+}
+
+int htr_max_mach(int dir, const htr_grid *g, const htr_fields *f, double *out, void *stream)
+{
+   if (!out) return set_error(HTR_E_ARG, "null argument");
+   if (dir < 0 || dir > 2) return set_error(HTR_E_ARG, "bad direction %d", dir);
+   Geo geo; htr_fields fr; int rc = prep_plain(g, f, fr, geo); if (rc) return done(rc);
+   HTR_NEED(velocity); HTR_NEED(SoS);
+   return done(max_mach_launch(dir, fr, geo, out, (hipStream_t)stream));
 }
 
 int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream)

@@ -4,5 +4,6 @@
 #include "euler.hip"
 #include "diffusion.hip"
 #include "pointwise.hip"
+#include "nscbc.hip"
 #include "chem.hip"
 #include "capi.hip"

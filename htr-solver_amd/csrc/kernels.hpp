@@ -63,6 +63,13 @@ template<class MX> int spectral_radius_launch(const htr_fields &f, const DMix *m
 // array (no allocation, no host read-back); n = cells of the rect (0: writes 0)
 template<class MX> int spectral_radius_noalloc_launch(const htr_fields &f, const DMix *mix, const Geo &g, int64_t n,
                                                       double *d_result, hipStream_t s);
+// CalculateMaxMachNumber along dir over the rect, read back to the host
+int max_mach_launch(int dir, const htr_fields &f, const Geo &g, double *result, hipStream_t s);
+// NSCBC corrections of Conserved_t in the boundary cells (minus: a Neg side)
+template<class MX> int nscbc_inflow_launch(int dir, int minus, const htr_fields &f, const DMix *mix, const Geo &g,
+                                           hipStream_t s);
+template<class MX> int nscbc_outflow_launch(int dir, int minus, const htr_fields &f, const DMix *mix, const Geo &g,
+                                            double MaxMach, double LengthScale, double PInf, hipStream_t s);
 int chem_set_launch(int block, int blocks_per_cu);
 int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, double dt, hipStream_t s);
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);

@@ -318,6 +318,26 @@ reduce_max_kernel(const double *__restrict__ partial, int n, double *__restrict_
    if (threadIdx.x == 0) *out = dmax(dmax(sm[0], sm[1]), dmax(sm[2], sm[3]));
 }
 
+// CalculateMaxMachNumber (src/prometeo_stat.rg:223-234): max |u_dir|/SoS,
+// grid-stride, one partial per block (the Mach number is >= 0, so the block
+// maxima start from 0)
+__global__ void __launch_bounds__(256)
+max_mach_kernel(const htr_fields f, const Geo g, int dir, int64_t n, double *__restrict__ partial)
+{
+   const int64_t tid = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
+   double r = 0.0;
+   for (int64_t i = tid; i < n; i += stride) {
+      const P3 p = exec_point(g, i);
+      r = dmax(r, fabs(ld<double>(f.velocity, p, dir))/ld<double>(f.SoS, p));
+   }
+   __shared__ double sm[4];
+   r = wave_max(r);
+   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+   __syncthreads();
+   if (threadIdx.x == 0) partial[blockIdx.x] = dmax(dmax(sm[0], sm[1]), dmax(sm[2], sm[3]));
+}
+
 // ------------------------------------------------------------------------ RK
 __global__ void __launch_bounds__(256)
 rk_update_kernel(const htr_fields f, const Geo g, int nEq, double c0, double c1, double c2,
@@ -460,6 +480,26 @@ int spectral_radius_launch(const htr_fields &f, const DMix *mix, const Geo &g, d
    if (hipMemcpyAsync(w.result_host, w.partial + Workspace::kPartial - 1, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
        hipStreamSynchronize(s) != hipSuccess)
       return set_error(HTR_E_DEVICE, "spectral radius: copy back failed");
+   *result = *w.result_host;
+   return HTR_OK;
+}
+
+int max_mach_launch(int dir, const htr_fields &f, const Geo &g, double *result, hipStream_t s)
+{
+   const int64_t n = exec_cells(g);
+   Workspace *wp = workspace(s);
+   if (!wp) return HTR_E_DEVICE;
+   Workspace &w = *wp;
+   const int64_t blocks = std::min<int64_t>(std::max<int64_t>(grid_for(n, 256), 1), 4096);
+   HTR_LAUNCH("max_mach", max_mach_kernel, dim3((unsigned)blocks), dim3(256), 0, s, f, g, dir, n, w.partial);
+   int rc = check_launch("max_mach_kernel");
+   if (rc) return rc;
+   HTR_LAUNCH("reduce_max", reduce_max_kernel, dim3(1), dim3(256), 0, s, w.partial, (int)blocks, w.partial + Workspace::kPartial - 1);
+   rc = check_launch("reduce_max_kernel");
+   if (rc) return rc;
+   if (hipMemcpyAsync(w.result_host, w.partial + Workspace::kPartial - 1, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess)
+      return set_error(HTR_E_DEVICE, "max Mach: copy back failed");
    *result = *w.result_host;
    return HTR_OK;
 }

@@ -43,7 +43,8 @@ class Fields(C.Structure):
                 ("pressure", View), ("temperature", View), ("MassFracs", View), ("MolarFracs", View),
                 ("velocity", View), ("Conserved", View), ("Conserved_old", View), ("Conserved_t", View),
                 ("Conserved_t_old", View), ("shockSensor", View * 3), ("DucrosSensor", View),
-                ("MolarFracs_profile", View), ("velocity_profile", View), ("temperature_profile", View)]
+                ("MolarFracs_profile", View), ("velocity_profile", View), ("temperature_profile", View),
+                ("dudtBoundary", View), ("dTdtBoundary", View)]
 
 
 class Diag(C.Structure):
@@ -60,7 +61,7 @@ EXPORTS = [
     "htr_rk_update", "htr_rk_init", "htr_full_rhs",
     "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
     "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl", "htr_set_bc",
-    "htr_max_spectral_radius_noalloc",
+    "htr_max_spectral_radius_noalloc", "htr_nscbc_inflow", "htr_nscbc_outflow", "htr_max_mach",
 ]
 
 _lib = None
@@ -91,6 +92,9 @@ def _declare(L):
         "htr_rk_stage": (i, [G, F, i, d, d, vp, vp, D, vp]),
         "htr_full_rhs_cfl": (i, [G, F, i, d, d, d, d, vp, i, vp, vp]),
         "htr_set_bc": (i, [i, i, G, F, d, vp, vp]),
+        "htr_nscbc_inflow": (i, [i, G, F, vp, vp]),
+        "htr_nscbc_outflow": (i, [i, G, F, d, d, d, vp, vp]),
+        "htr_max_mach": (i, [i, G, F, dp, vp]),
         "htr_mix_create_constprop": (i, [C.POINTER(vp), d, d, d, i, d, d, d]),
         "htr_mix_create_airmix": (i, [C.POINTER(vp), d, d, d, i, ip, dp]),
         "htr_mix_create_species": (i, [C.POINTER(vp), i, dp, fp, ip, dp, dp, dp, ip, dp]),

@@ -23,7 +23,8 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "max_spectral_radius_noalloc",
            "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report",
            "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve", "set_bc",
-           "BC_SIDES"]
+           "BC_SIDES", "update_using_flux_nscbc_inflow", "update_using_flux_nscbc_outflow",
+           "calculate_max_mach_number"]
 
 
 def stream_ptr(stream=None):
@@ -117,7 +118,8 @@ def make_grid(n, lo=(0, 0, 0), hi=None, bounds_lo=(0, 0, 0)):
 SCALARS = ("rho", "mu", "lam", "SoS", "pressure", "temperature", "DucrosSensor")
 VECTORS = {"Di": "ns", "MassFracs": "ns", "MolarFracs": "ns", "velocity": 3,
            "Conserved": "neq", "Conserved_old": "neq", "Conserved_t": "neq", "Conserved_t_old": "neq"}
-PROFILES = {"MolarFracs_profile": "ns", "velocity_profile": 3, "temperature_profile": 0}
+PROFILES = {"MolarFracs_profile": "ns", "velocity_profile": 3, "temperature_profile": 0,
+            "dudtBoundary": 3, "dTdtBoundary": 0}
 
 
 class FieldSet:
@@ -321,6 +323,32 @@ def set_bc(fs, kind, side, grid, mix, Pbc=0.0, stream=None):
     BC_SIDES; grid's execution rect is the boundary's ghost cells."""
     sd = BC_SIDES.index(side) if isinstance(side, str) else int(side)
     check(lib().htr_set_bc(int(kind), sd, C.byref(grid), C.byref(fs.c()), float(Pbc), mix.h, stream_ptr(stream)))
+
+
+def _side(side):
+    return BC_SIDES.index(side) if isinstance(side, str) else int(side)
+
+
+def update_using_flux_nscbc_inflow(fs, side, mix, grid, stream=None):
+    """UpdateUsingFluxNSCBCInflow{Minus,Plus}Side: LODI corrections of
+    Conserved_t in the boundary cells of one side (grid's execution rect),
+    reading the inflow profile's dudtBoundary and dTdtBoundary."""
+    check(lib().htr_nscbc_inflow(_side(side), C.byref(grid), C.byref(fs.c()), mix.h, stream_ptr(stream)))
+
+
+def update_using_flux_nscbc_outflow(fs, side, mix, MaxMach, LengthScale, PInf, grid, stream=None):
+    """UpdateUsingFluxNSCBCOutflow{Minus,Plus}Side: LODI corrections of
+    Conserved_t in the boundary cells of one side, the incoming wave relaxed
+    towards PInf over LengthScale; MaxMach from calculate_max_mach_number."""
+    check(lib().htr_nscbc_outflow(_side(side), C.byref(grid), C.byref(fs.c()), float(MaxMach), float(LengthScale),
+                                  float(PInf), mix.h, stream_ptr(stream)))
+
+
+def calculate_max_mach_number(fs, direction, grid=None, stream=None):
+    """CalculateMaxMachNumber: max |u_dir|/SoS over the rect (host value)."""
+    out = C.c_double(0.0)
+    check(lib().htr_max_mach(int(direction), _g(fs, grid), C.byref(fs.c()), C.byref(out), stream_ptr(stream)))
+    return out.value
 
 
 def workspace_reserve(fs, mix, grid=None, stream=None):

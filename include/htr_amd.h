@@ -96,6 +96,9 @@ typedef struct {
    htr_view DucrosSensor;
    /* boundary profiles read by the ghost-cell tasks (src/prometeo_types.h ProfilesVars) */
    htr_view MolarFracs_profile, velocity_profile, temperature_profile;
+   /* time derivatives of the inflow profile read by the NSCBC inflow task
+    * (src/prometeo_types.h dudtBoundary, dTdtBoundary) */
+   htr_view dudtBoundary, dTdtBoundary;
 } htr_fields;
 
 /* Optional diagnostics written by instrumented calls (any view may be null):
@@ -195,6 +198,30 @@ int htr_shock_sensor(int dir, const htr_grid *g, const htr_fields *f, const htr_
  * the same rect. */
 int htr_set_bc(int kind, int side, const htr_grid *g, const htr_fields *f, double Pbc, const htr_mix *m,
                void *stream);
+
+/* ---- NSCBC boundary fluxes (src/prometeo_rhs.inl:1288-1631) ---------------- */
+/* Characteristic (LODI) corrections of Conserved_t in the boundary cells of
+ * one side: g->exec is the boundary rect (the BC region of the task), side
+ * 0..5 = xNeg, xPos, yNeg, yPos, zNeg, zPos; the interior point of a
+ * boundary cell is one cell inward along the normal and must lie in
+ * Fluid_bounds.
+ * UpdateUsingFluxNSCBCInflow{Minus,Plus}Side<dir>::gpu_base_impl
+ *   (src/prometeo_rhs.cu:667-711, 774-818): reads MassFracs, pressure,
+ *   velocity, rho, SoS, temperature, dudtBoundary, dTdtBoundary and the
+ *   normal nType/m_d. */
+int htr_nscbc_inflow(int side, const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
+/* UpdateUsingFluxNSCBCOutflow{Minus,Plus}Side<dir>::gpu_base_impl
+ *   (src/prometeo_rhs.cu:910-986, 1061-1136): MaxMach is the task's future
+ *   (the maximum Mach number along the normal over the interior,
+ *   htr_max_mach), LengthScale and PInf its arguments; reads nType and m_d of
+ *   all three directions, MassFracs, rho, mu, pressure, velocity, SoS,
+ *   temperature, Conserved. */
+int htr_nscbc_outflow(int side, const htr_grid *g, const htr_fields *f, double MaxMach, double LengthScale,
+                      double PInf, const htr_mix *m, void *stream);
+/* CalculateMaxMachNumber (src/prometeo_stat.rg:223-234): max |u_dir|/SoS over
+ * g->exec into *out; waits for the stream.  An empty rect gives 0 (the
+ * reference starts from -inf, but its tiles always hold cells). */
+int htr_max_mach(int dir, const htr_grid *g, const htr_fields *f, double *out, void *stream);
 
 /* ---- Euler fluxes (src/prometeo_rhs.hpp:171-366) -------------------------- */
 /* scheme HTR_EULER_HYBRID: UpdateUsingHybridEulerFluxTask<dir>::gpu_base_impl

@@ -121,6 +121,7 @@ typedef struct {
    uint8_t *shockSensor[3];
    double *DucrosSensor;
    double *MolarFracs_profile, *velocity_profile, *temperature_profile;
+   double *dudtBoundary, *dTdtBoundary;
 } o_fields;
 
 /* ---- mixture builders --------------------------------------------------- */
@@ -215,6 +216,15 @@ double o_max_spectral_radius(const o_grid *g, const o_fields *f, const o_mix *m)
  * kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC; side
  * 0..5 = xNeg, xPos, yNeg, yPos, zNeg, zPos; the rect of g = the ghost cells */
 void   o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc, const o_mix *m);
+/* NSCBC flux corrections of the boundary cells of one side (the rect of g,
+ * side 0..5 as above): UpdateUsingFluxNSCBCInflow{Minus,Plus}Side
+ * (src/prometeo_rhs.inl:1288-1420) and UpdateUsingFluxNSCBCOutflow{Minus,Plus}Side
+ * (src/prometeo_rhs.inl:1424-1631); Conserved_t updated in place */
+void   o_nscbc_inflow(const o_grid *g, const o_fields *f, int side, const o_mix *m);
+void   o_nscbc_outflow(const o_grid *g, const o_fields *f, int side, double MaxMach, double LengthScale,
+                       double PInf, const o_mix *m);
+/* max |u_dir|/SoS over the rect of g (CalculateMaxMachNumber, src/prometeo_stat.rg:223-234) */
+double o_max_mach(const o_grid *g, const o_fields *f, int dir);
 /* SSP-RK3 update of Conserved (src/prometeo_rk.rg:88-112) */
 void   o_rk_update(const o_grid *g, const o_fields *f, int nEq, int stage, double dt);
 

@@ -629,3 +629,213 @@ void o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc
             }
          }
 }
+
+/* ------------------------------------------------------------------------
+ * NSCBC (characteristic) flux corrections of the boundary cells.
+ *   src/prometeo_rhs.inl:1288-1352  InflowMinusSide::addLODIfluxes
+ *   src/prometeo_rhs.inl:1354-1420  InflowPlusSide::addLODIfluxes
+ *   src/prometeo_rhs.inl:1424-1525  OutflowMinusSide::addLODIfluxes
+ *   src/prometeo_rhs.inl:1527-1631  OutflowPlusSide::addLODIfluxes
+ *   src/prometeo_rhs.cc:529-575, 680-730  task loops over the BC rect
+ *   src/prometeo_metric.inl:89-152  getDeriv, getDerivLeftBC, getDerivRightBC
+ *   src/prometeo_stat.rg:223-234    CalculateMaxMachNumber
+ *   src/Utils/PointDomain_helper.hpp:76-134  tangential directions, p_int
+ * The interior point of boundary cell p is p + 1 along the normal on a Minus
+ * side, p - 1 on a Plus side.  Derivatives at a point wrap periodically in
+ * the box, as warpPeriodic does over Fluid_bounds.
+ * Pinned by tests/test_oracle_pins.py (closed forms of reduced states: a
+ * uniform state at the far-field pressure leaves the RHS unchanged, a
+ * pressure offset gives the relaxation wave, a supersonic inlet clears the
+ * RHS, prescribed dT/dt and du/dt give the inlet waves, a normal pressure
+ * gradient gives the outgoing acoustic wave; every side and direction).
+ */
+static inline double deriv_left(int t, double x, double xp1, double m)
+{ return ((double)o_coeff_Grad(t, 1)*(xp1 - x))*m; }
+static inline double deriv_right(int t, double xm1, double x, double m)
+{ return ((double)o_coeff_Grad(t, 0)*(x - xm1))*m; }
+
+/* getDeriv<d>(q, p, comp, nType_d[p], m_d_d[p], bounds) on a component field */
+static double deriv_at(const o_grid *g, const o_fields *f, const double *q, int d, const int *c)
+{
+   const long p = L(g, c[0], c[1], c[2]);
+   const int t = f->nType[d][p];
+   const double m = f->m_d[d][p];
+   int cm[3] = { c[0], c[1], c[2] }, cp[3] = { c[0], c[1], c[2] };
+   const int n = d == 0 ? g->nx : d == 1 ? g->ny : g->nz;
+   cm[d] = wrapm(cm[d], o_coeff_Cp(t, 1), n);
+   cp[d] = wrapp(cp[d], o_coeff_Cp(t, 2), n);
+   const double xm1 = q[L(g, cm[0], cm[1], cm[2])], x = q[p], xp1 = q[L(g, cp[0], cp[1], cp[2])];
+   return ((double)o_coeff_Grad(t, 0)*(x - xm1) + (double)o_coeff_Grad(t, 1)*(xp1 - x))*m;
+}
+
+static inline double species_W(const o_mix *m, int s) { return m->constprop ? 8.3144598/m->R : m->sp[s].W; }
+
+void o_nscbc_inflow(const o_grid *g, const o_fields *f, int side, const o_mix *m)
+{
+   const int ns = m->nSpec, neq = ns + 4;
+   const int dir = side/2, minus = (side % 2 == 0);
+   const long N = NN(g);
+   const double *vN = f->velocity + dir*N;
+   for (int k = g->lo[2]; k <= g->hi[2]; k++)
+      for (int j = g->lo[1]; j <= g->hi[1]; j++)
+         for (int i = g->lo[0]; i <= g->hi[0]; i++) {
+            int c[3] = { i, j, k };
+            const long p = L(g, i, j, k);
+            c[dir] += minus ? 1 : -1;
+            const long q = L(g, c[0], c[1], c[2]);
+            const double SoS = f->SoS[p], rho = f->rho[p], T = f->temperature[p];
+            const double un = vN[p];
+            if (minus ? (un >= SoS) : (un <= -SoS)) {
+               /* supersonic inlet */
+               for (int l = 0; l < neq; l++) f->Conserved_t[l*N + p] = 0.0;
+               continue;
+            }
+            double Y[O_NSP_MAX], LS[O_NSP_MAX];
+            for (int s = 0; s < ns; s++) Y[s] = f->MassFracs[s*N + p];
+            const int t = f->nType[dir][p];
+            const double md = f->m_d[dir][p];
+            const double MixW = o_GetMolarWeightFromYi(m, Y);
+            const double Cp = o_GetHeatCapacity(m, T, Y);
+            const double lambda = un;
+            double L1, LN;
+            if (minus) {
+               const double lambda_1 = un - SoS;
+               const double dp_dn = deriv_left(t, f->pressure[p], f->pressure[q], md);
+               const double du_dn = deriv_left(t, vN[p], vN[q], md);
+               L1 = lambda_1*(dp_dn - rho*SoS*du_dn);
+               for (int s = 0; s < ns; s++)
+                  LS[s] = lambda*deriv_left(t, f->MassFracs[s*N + p], f->MassFracs[s*N + q], md);
+               LN = L1 - 2*rho*SoS*f->dudtBoundary[dir*N + p];
+            } else {
+               const double lambda_N = un + SoS;
+               const double dp_dn = deriv_right(t, f->pressure[q], f->pressure[p], md);
+               const double du_dn = deriv_right(t, vN[q], vN[p], md);
+               LN = lambda_N*(dp_dn + rho*SoS*du_dn);
+               for (int s = 0; s < ns; s++)
+                  LS[s] = lambda*deriv_right(t, f->MassFracs[s*N + q], f->MassFracs[s*N + p], md);
+               L1 = LN + 2*rho*SoS*f->dudtBoundary[dir*N + p];
+            }
+            double L2 = f->dTdtBoundary[p]/T + (LN + L1)/(2*rho*Cp*T);
+            for (int s = 0; s < ns; s++) L2 -= MixW/species_W(m, s)*LS[s];
+            L2 *= -rho*SoS*SoS;
+            const double d1 = (0.5*(L1 + LN) - L2)/(SoS*SoS);
+            for (int s = 0; s < ns; s++) f->Conserved_t[s*N + p] -= (d1*Y[s] + rho*LS[s]);
+         }
+}
+
+void o_nscbc_outflow(const o_grid *g, const o_fields *f, int side, double MaxMach, double LengthScale,
+                     double PInf, const o_mix *m)
+{
+   const int ns = m->nSpec;
+   const int dir = side/2, minus = (side % 2 == 0);
+   const int iN = dir, iT1 = (dir == 0) ? 1 : 0, iT2 = (dir == 2) ? 1 : 2;
+   const int irU = ns, irE = ns + 3;
+   const long N = NN(g);
+   const double *v[3] = { f->velocity, f->velocity + N, f->velocity + 2*N };
+   for (int k = g->lo[2]; k <= g->hi[2]; k++)
+      for (int j = g->lo[1]; j <= g->hi[1]; j++)
+         for (int i = g->lo[0]; i <= g->hi[0]; i++) {
+            const int cb[3] = { i, j, k };
+            int ci[3] = { i, j, k };
+            const long p = L(g, i, j, k);
+            ci[dir] += minus ? 1 : -1;
+            const long q = L(g, ci[0], ci[1], ci[2]);
+            const int t = f->nType[iN][p];
+            const double md = f->m_d[iN][p];
+            const double SoS = f->SoS[p], T = f->temperature[p], rho = f->rho[p], P = f->pressure[p];
+            const double un = v[iN][p];
+            double Y[O_NSP_MAX], LS[O_NSP_MAX];
+            for (int s = 0; s < ns; s++) Y[s] = f->MassFracs[s*N + p];
+            double dp_dn, dun_dn, duT1dn, duT2dn, drho_dn;
+            if (minus) {
+               dp_dn  = deriv_left(t, P, f->pressure[q], md);
+               dun_dn = deriv_left(t, un, v[iN][q], md);
+               duT1dn = deriv_left(t, v[iT1][p], v[iT1][q], md);
+               duT2dn = deriv_left(t, v[iT2][p], v[iT2][q], md);
+               drho_dn = deriv_left(t, rho, f->rho[q], md);
+            } else {
+               dp_dn  = deriv_right(t, f->pressure[q], P, md);
+               dun_dn = deriv_right(t, v[iN][q], un, md);
+               duT1dn = deriv_right(t, v[iT1][q], v[iT1][p], md);
+               duT2dn = deriv_right(t, v[iT2][q], v[iT2][p], md);
+               drho_dn = deriv_right(t, f->rho[q], rho, md);
+            }
+            const double lambda_1 = un - SoS;
+            const double lambda   = un;
+            const double lambda_s = minus ? dmin(un, 0.0) : dmax(un, 0.0);
+            const double lambda_N = un + SoS;
+            const double sigma = 0.25;
+            const double K = (MaxMach < 0.99) ? sigma*(1.0 - MaxMach*MaxMach)*SoS/LengthScale
+                                              : sigma*(SoS - (un*un)/SoS)/LengthScale;
+            double L1, LN;
+            if (minus) {
+               L1 = lambda_1*(dp_dn - rho*SoS*dun_dn);
+               LN = (lambda_N < 0) ? lambda_N*(dp_dn + rho*SoS*dun_dn) : K*(P - PInf);
+            } else {
+               L1 = (lambda_1 > 0) ? lambda_1*(dp_dn - rho*SoS*dun_dn) : K*(P - PInf);
+               LN = lambda_N*(dp_dn + rho*SoS*dun_dn);
+            }
+            const double LM = lambda*(dp_dn - SoS*SoS*drho_dn);
+            for (int s = 0; s < ns; s++)
+               LS[s] = lambda_s*(minus ? deriv_left(t, Y[s], f->MassFracs[s*N + q], md)
+                                       : deriv_right(t, f->MassFracs[s*N + q], Y[s], md));
+            const double d1 = (0.5*(L1 + LN) - LM)/(SoS*SoS);
+            double dM[3];
+            dM[iN]  = (LN - L1)/(2*rho*SoS);
+            dM[iT1] = lambda*duT1dn;
+            dM[iT2] = lambda*duT2dn;
+            const double dN = LM/(SoS*SoS);
+            /* viscous terms */
+            double vGradT1[3], vGradT2[3];
+            for (int c = 0; c < 3; c++) {
+               vGradT1[c] = deriv_at(g, f, v[c], iT1, cb);
+               vGradT2[c] = deriv_at(g, f, v[c], iT2, cb);
+            }
+            const double tauNN_bnd = f->mu[p]*(4*dun_dn - 2*vGradT1[iT1] - 2*vGradT2[iT2])/3;
+            const double tauNN_int = f->mu[q]*(4*deriv_at(g, f, v[iN], iN, ci)
+                                               - 2*deriv_at(g, f, v[iT1], iT1, ci)
+                                               - 2*deriv_at(g, f, v[iT2], iT2, ci))/3;
+            double dtau_dn, vh;
+            if (minus) {
+               dtau_dn = deriv_left(t, tauNN_bnd, tauNN_int, md);
+               vh = deriv_left(t, un*tauNN_bnd, v[iN][q]*tauNN_int, md);
+            } else {
+               dtau_dn = deriv_right(t, tauNN_int, tauNN_bnd, md);
+               vh = deriv_right(t, v[iN][q]*tauNN_int, un*tauNN_bnd, md);
+            }
+            const double viscous_heating = vh + duT1dn*f->mu[p]*(duT1dn + vGradT1[iN]) +
+                                           duT2dn*f->mu[p]*(duT2dn + vGradT2[iN]);
+            /* RHS update */
+            double *Ct = f->Conserved_t;
+            const double *Cs = f->Conserved;
+            const double cpT = o_GetHeatCapacity(m, T, Y)*T;
+            for (int s = 0; s < ns; s++) Ct[s*N + p] -= (d1*Y[s] + rho*LS[s]);
+            Ct[(irU+iN)*N + p]  -= (v[iN][p]*d1 + rho*dM[iN] - dtau_dn);
+            Ct[(irU+iT1)*N + p] -= (v[iT1][p]*d1 + rho*dM[iT1]);
+            Ct[(irU+iT2)*N + p] -= (v[iT2][p]*d1 + rho*dM[iT2]);
+            Ct[irE*N + p] -= ((Cs[irE*N + p] + P)*d1/rho
+                              + Cs[(irU+iN)*N + p]*dM[iN]
+                              + Cs[(irU+iT1)*N + p]*dM[iT1]
+                              + Cs[(irU+iT2)*N + p]*dM[iT2]
+                              + cpT*dN
+                              - viscous_heating);
+            const double cpT_Wmix = cpT*o_GetMolarWeightFromYi(m, Y);
+            for (int s = 0; s < ns; s++) {
+               const double dEdYi = o_GetSpeciesEnthalpy(m, s, T) - cpT_Wmix/species_W(m, s);
+               Ct[irE*N + p] -= (rho*LS[s]*dEdYi);
+            }
+         }
+}
+
+double o_max_mach(const o_grid *g, const o_fields *f, int dir)
+{
+   const long N = NN(g);
+   double acc = -HUGE_VAL;
+   for (int k = g->lo[2]; k <= g->hi[2]; k++)
+      for (int j = g->lo[1]; j <= g->hi[1]; j++)
+         for (int i = g->lo[0]; i <= g->hi[0]; i++) {
+            const long p = L(g, i, j, k);
+            acc = dmax(acc, fabs(f->velocity[dir*N + p])/f->SoS[p]);
+         }
+   return acc;
+}

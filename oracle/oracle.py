@@ -57,7 +57,8 @@ class OFields(C.Structure):
                 ("Conserved", _dp), ("Conserved_old", _dp), ("Conserved_t", _dp),
                 ("Conserved_t_old", _dp),
                 ("shockSensor", _u8p * 3), ("DucrosSensor", _dp),
-                ("MolarFracs_profile", _dp), ("velocity_profile", _dp), ("temperature_profile", _dp)]
+                ("MolarFracs_profile", _dp), ("velocity_profile", _dp), ("temperature_profile", _dp),
+                ("dudtBoundary", _dp), ("dTdtBoundary", _dp)]
 
 
 def _declare(L):
@@ -121,6 +122,9 @@ def _declare(L):
         "o_max_spectral_radius": (d, [C.POINTER(OGrid), C.POINTER(OFields), vp]),
         "o_rk_update": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, i, d]),
         "o_set_bc": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, i, d, vp]),
+        "o_nscbc_inflow": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, vp]),
+        "o_nscbc_outflow": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, d, d, d, vp]),
+        "o_max_mach": (d, [C.POINTER(OGrid), C.POINTER(OFields), i]),
         "o_full_rhs": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, d, d, d, i, vp]),
         "o_lu_solve": (None, [i, _dp, _dp]),
         "o_rosenbrock_testprob": (None, [_dp, d, d, C.POINTER(C.c_int32)]),
@@ -295,6 +299,7 @@ FIELD_COMPONENTS = {
     "Di": "ns", "MassFracs": "ns", "MolarFracs": "ns", "velocity": 3,
     "Conserved": "neq", "Conserved_old": "neq", "Conserved_t": "neq", "Conserved_t_old": "neq",
     "MolarFracs_profile": "ns", "velocity_profile": 3, "temperature_profile": 0,
+    "dudtBoundary": 3, "dTdtBoundary": 0,
 }
 
 
@@ -389,6 +394,21 @@ def max_spectral_radius(g, f, m):
 
 def set_bc(g, f, kind, side, Pbc, m):
     lib().o_set_bc(C.byref(g.c()), C.byref(f.c()), kind, side, Pbc, m.ptr)
+
+
+def nscbc_inflow(g, f, side, m):
+    """UpdateUsingFluxNSCBCInflow{Minus,Plus}Side over the rect of g (side 0..5)."""
+    lib().o_nscbc_inflow(C.byref(g.c()), C.byref(f.c()), side, m.ptr)
+
+
+def nscbc_outflow(g, f, side, MaxMach, LengthScale, PInf, m):
+    """UpdateUsingFluxNSCBCOutflow{Minus,Plus}Side over the rect of g (side 0..5)."""
+    lib().o_nscbc_outflow(C.byref(g.c()), C.byref(f.c()), side, MaxMach, LengthScale, PInf, m.ptr)
+
+
+def max_mach(g, f, d):
+    """CalculateMaxMachNumber over the rect of g."""
+    return lib().o_max_mach(C.byref(g.c()), C.byref(f.c()), d)
 
 
 def rk_update(g, f, nEq, stage, dt):

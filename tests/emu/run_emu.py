@@ -22,7 +22,8 @@ import torch  # noqa: E402
 import htr_amd as H  # noqa: E402
 import htr_amd.api as A  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from hotpath import air_state, complete_state, gpu_mix, oracle_fields, tgv_state, wall_state, dt_cfl  # noqa: E402
+from hotpath import air_state, complete_state, gpu_mix, nscbc_state, oracle_fields, tgv_state, wall_state, \
+    dt_cfl  # noqa: E402
 
 # host tensors stand for device memory; the emulated library ignores streams
 A.stream_ptr = lambda stream=None: ctypes.c_void_p(0)
@@ -66,7 +67,7 @@ def euler(st, meta, n, om, d, scheme, label, RK=(1.0, 0.0), dt=1e-3, lo=(0, 0, 0
 def main():
     H.init(0)
     t0 = time.time()
-    which = sys.argv[1:] or ["euler", "limiter", "diffusion", "rhs", "stage", "wall", "bc"]
+    which = sys.argv[1:] or ["euler", "limiter", "diffusion", "rhs", "stage", "wall", "bc", "nscbc"]
 
     st, meta = air_state(8, seed=4)
     n = (8, 8, 8)
@@ -206,6 +207,29 @@ def main():
                 H.update_conserved_from_primitive(fs, gpu_mix(meta), grid=grid)
                 for k in ("temperature", "pressure", "rho", "Conserved"):
                     report("bc%d_kind%d_%s" % (side, kind, k), fs[k].numpy(), f[k], 64)
+    if "nscbc" in which:
+        sn, metan, nn = nscbc_state(6)
+        sn, omn = complete_state(sn, metan, nn, sensors=False)
+        gmn = gpu_mix(metan)
+        for side in range(6):
+            d, neg = side // 2, side % 2 == 0
+            lo, hi = [0, 0, 0], [5, 5, 5]
+            lo[d] = hi[d] = 0 if neg else 5
+            g, f = oracle_fields(sn, 5, nn)
+            fs = to_host_fs(sn, 5, nn, profiles=True)
+            mm = O.max_mach(g, f, d)
+            report("max_mach_d%d_side%d" % (d, side), H.calculate_max_mach_number(fs, d), mm, 216)
+            for kind, Mmax in (("inflow", None), ("outflow", 0.5), ("outflow_sup", max(mm, 1.0))):
+                f["Conserved_t"] = 0.0
+                fs["Conserved_t"] = 0.0
+                gr, grid = g.with_rect(tuple(lo), tuple(hi)), fs.grid(tuple(lo), tuple(hi))
+                if Mmax is None:
+                    O.nscbc_inflow(gr, f, side, omn)
+                    H.update_using_flux_nscbc_inflow(fs, side, gmn, grid)
+                else:
+                    O.nscbc_outflow(gr, f, side, Mmax, 0.8, 0.95, omn)
+                    H.update_using_flux_nscbc_outflow(fs, side, gmn, Mmax, 0.8, 0.95, grid)
+                report("nscbc_%s_side%d" % (kind, side), fs["Conserved_t"].numpy(), f["Conserved_t"], 36)
     print("DONE %.1f s" % (time.time() - t0), flush=True)
 
 

@@ -139,3 +139,27 @@ def wall_state(nx, ny, nz, seed=3, collocated=False):
     pz = cases.periodic_metrics(nz)
     cases.set_metrics(out, [px, (nt, me, md, ms), pz], n)
     return out, meta, n
+
+
+def nscbc_state(n, seed=5):
+    """Air box bounded by collocated boundary nodes (L_C .. R_C) in all three
+    directions on stretched grids, with a mean flow through it, a few
+    supersonic cells on every face and random inflow-profile time
+    derivatives (dudtBoundary, dTdtBoundary): the inputs of the NSCBC tasks."""
+    st, meta = cases.air_box(n, seed, composition="per-cell")
+    rng = np.random.default_rng(seed)
+    dims = [cases.wall_metrics(n - 2, width=1.0, stretching=1.2 + 0.1 * d, collocated=True)[:4] for d in range(3)]
+    cases.set_metrics(st, dims, (n, n, n))
+    st = {k: np.ascontiguousarray(v) for k, v in st.items()}
+    u = st["velocity"]
+    scale = np.sqrt(np.mean(u * u))
+    u = 0.5 * u + np.array([0.9, -0.6, 0.4])[:, None, None, None] * scale
+    # supersonic spots (mach ~3 along each direction)
+    for d in range(3):
+        idx = rng.integers(0, n, size=(6, 3))
+        for z, y, x in idx:
+            u[d, z, y, x] = (1 if rng.random() < 0.5 else -1) * 6.0 * scale
+    st["velocity"] = u
+    st["dudtBoundary"] = rng.normal(size=(3, n, n, n)) * scale
+    st["dTdtBoundary"] = rng.normal(size=(n, n, n)) * 0.1 * np.mean(st["temperature"])
+    return st, meta, (n, n, n)

@@ -65,6 +65,6 @@ def test_teno_pipeline_over_many_chunks(emu_lib):
 
 
 def test_diffusion_rhs_stage_and_ghosts(emu_lib):
-    res = run_cases(emu_lib, "diffusion", "rhs", "stage", "bc")
+    res = run_cases(emu_lib, "diffusion", "rhs", "stage", "bc", "nscbc")
     assert len(res) >= 60
     assert all(v == 0.0 for v in res.values()), res

@@ -15,8 +15,8 @@ import pytest
 
 from oracle import oracle as O
 from htr_amd import cases
-from hotpath import TOL, air_state, complete_state, dt_cfl, flux_scale, gpu_mix, norm_err, oracle_fields, \
-    tgv_state, to_gpu, wall_state
+from hotpath import TOL, air_state, complete_state, dt_cfl, flux_scale, gpu_mix, norm_err, nscbc_state, \
+    oracle_fields, tgv_state, to_gpu, wall_state
 
 pytestmark = pytest.mark.gpu
 
@@ -660,3 +660,62 @@ def test_bc_ghosts(side):
             e = norm_err(host(fs[k]), f[k])
             assert max(e) <= TOL, (kind, k, e)
     say("  kinds 0-2: ghost primitives bit-identical, properties/conserved within 1e-12")
+
+
+@pytest.fixture(scope="module")
+def nscbc10():
+    st, meta, n = nscbc_state(10)
+    st, om = complete_state(st, meta, n, sensors=False)
+    return st, meta, n, om
+
+
+def side_rect(n, side):
+    d = side // 2
+    lo, hi = [0, 0, 0], [x - 1 for x in n]
+    lo[d] = hi[d] = 0 if side % 2 == 0 else n[d] - 1
+    return tuple(lo), tuple(hi)
+
+
+@pytest.mark.parametrize("side", range(6))
+def test_nscbc_inflow_outflow(nscbc10, side):
+    """NSCBC inflow and outflow corrections of the boundary cells of every
+    side against the oracle (collocated boundary nodes, stretched grids,
+    subsonic and supersonic cells, backflow), both relaxation branches of
+    the outflow (MaxMach below and above 0.99)."""
+    import htr_amd as H
+    st, meta, n, om = nscbc10
+    lo, hi = side_rect(n, side)
+    gm = gpu_mix(meta)
+    g, f = oracle_fields(st, 5, n)
+    fs = to_gpu(st, 5, n, profiles=True)
+    mm = O.max_mach(g, f, side // 2)
+    got = H.calculate_max_mach_number(fs, side // 2)
+    say("case nscbc side %d: max Mach oracle %.17g gpu %.17g" % (side, mm, got))
+    assert got == mm
+    runs = (("inflow", None), ("outflow M<0.99", 0.4), ("outflow M>=0.99", max(mm, 1.0)))
+    for label, Mmax in runs:
+        f["Conserved_t"] = 0.0
+        fs["Conserved_t"] = 0.0
+        gg = g.with_rect(lo, hi)
+        grid = fs.grid(lo, hi)
+        if Mmax is None:
+            O.nscbc_inflow(gg, f, side, om)
+            H.update_using_flux_nscbc_inflow(fs, side, gm, grid)
+        else:
+            PInf = float(np.mean(st["pressure"])) * 0.97
+            O.nscbc_outflow(gg, f, side, Mmax, 0.8, PInf, om)
+            H.update_using_flux_nscbc_outflow(fs, side, gm, Mmax, 0.8, PInf, grid)
+        ref = f["Conserved_t"]
+        gpu = host(fs["Conserved_t"])
+        e = norm_err(gpu, ref)
+        cleared = int(np.sum(np.all(ref == 0.0, axis=0)))
+        say("  %s: max norm err %.3e, cells with a cleared RHS %d" % (label, max(e), cleared))
+        assert np.any(ref != 0.0)
+        assert max(e) <= TOL, (label, e)
+        # nothing outside the boundary rect changes
+        mask = np.ones(ref.shape[1:], bool)
+        d = side // 2
+        sl = [slice(None)] * 3
+        sl[2 - d] = lo[d]
+        mask[tuple(sl)] = False
+        assert np.all(gpu[:, mask] == 0.0)

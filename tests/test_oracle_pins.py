@@ -34,6 +34,17 @@ a swapped stencil row, a wrong blend -- fails it:
                      Lax-Friedrichs flux built independently from the
                      projections, and the predicted partial densities on both
                      sides are >= min(1e-60, first-order prediction).
+  NSCBC              reduced states of the characteristic boundary fluxes
+                     (src/prometeo_rhs.inl:1288-1631), each side of each
+                     direction: a uniform state at the far-field pressure
+                     leaves the RHS unchanged; a pressure offset gives the
+                     relaxation wave K (p - PInf) of both K branches; a
+                     normal pressure gradient gives the outgoing acoustic
+                     wave; a tangential shear gives the viscous heating
+                     mu (du_t/dn)^2; a mass-fraction gradient convected out
+                     gives rho u dY/dn and its enthalpy term, and nothing when
+                     the flow enters; a supersonic inlet clears the RHS;
+                     prescribed dT/dt and du_n/dt give the inlet waves.
 """
 import numpy as np
 import pytest
@@ -343,3 +354,207 @@ def test_bc_ghost_values(side):
             a, b = f[k], before[k]
             keep = [i for i in range(n[d]) if i != gi]
             assert np.all(np.take(a, keep, axis=a.ndim - 3 + ax) == np.take(b, keep, axis=b.ndim - 3 + ax))
+
+
+# ---------------------------------------------------------------- NSCBC
+NSCBC_N = (6, 7, 5)
+
+
+def nscbc_case(side, seed=0):
+    """Air fields on a small box with collocated boundary node types along the
+    normal of the side (L_C .. R_C, Grad rows (0, 1) and (1, 0) at the ends)
+    and a uniform reference state; returns (grid on the boundary layer of
+    the side, fields, mixture, helpers)."""
+    n = NSCBC_N
+    d, minus = side // 2, side % 2 == 0
+    st, meta = air_state(8, seed=1)
+    om = oracle_mix(meta)
+    g = O.Grid(*n)
+    f = O.Fields(g, 5)
+    rng = np.random.default_rng(seed + 17 * side)
+    Y = np.array([0.70, 0.20, 0.05, 0.03, 0.02])
+    f["MassFracs"] = Y[:, None, None, None]
+    f["rho"], f["SoS"], f["temperature"], f["pressure"], f["mu"] = 1.3, 1.7, 1.1, 0.9, 2e-3
+    f["velocity"] = 0.0
+    f["Conserved"] = rng.uniform(0.5, 1.5, size=f["Conserved"].shape)
+    f["Conserved_t"] = rng.uniform(-1.0, 1.0, size=f["Conserved_t"].shape)
+    nt = np.full(n[d], cases.STD, np.int32)
+    nt[0], nt[1] = cases.L_C, cases.LP1_C
+    nt[-1], nt[-2], nt[-3] = cases.R_C, cases.RM1_C, cases.RM2_C
+    shape = [1, 1, 1]
+    shape[2 - d] = n[d]
+    f["nType%d" % d] = nt.reshape(shape)
+    for dd in range(3):
+        f["m_d%d" % dd] = 2.5 if dd == d else 1.3
+    lo, hi = [0, 0, 0], [n[0] - 1, n[1] - 1, n[2] - 1]
+    lo[d] = hi[d] = 0 if minus else n[d] - 1
+    ax = 2 - d
+    bi, ii = (0, 1) if minus else (n[d] - 1, n[d] - 2)
+
+    def bnd(a):
+        return np.take(a, bi, axis=a.ndim - 3 + ax)
+
+    def along(x):
+        """a field linear along the normal: x * index"""
+        sh = [1, 1, 1]
+        sh[ax] = n[d]
+        return (np.arange(n[d], dtype=np.float64) * x).reshape(sh)
+
+    return g.with_rect(tuple(lo), tuple(hi)), f, om, dict(d=d, minus=minus, bnd=bnd, along=along, ii=ii, bi=bi)
+
+
+def nscbc_expect(f0, h, ns, d1, dM, rhoLS=None, dE_extra=0.0, dN=0.0, cpT=0.0):
+    """Conserved_t at the boundary cells after the LODI update with the given
+    wave terms (the reduced form of the outflow update)."""
+    d = h["d"]
+    bnd = h["bnd"]
+    Ct = bnd(f0["Conserved_t"]).copy()
+    Y = bnd(f0["MassFracs"])
+    rho, P = bnd(f0["rho"]), bnd(f0["pressure"])
+    u = bnd(f0["velocity"])
+    C = bnd(f0["Conserved"])
+    for s in range(ns):
+        Ct[s] -= d1 * Y[s] + (rhoLS[s] if rhoLS is not None else 0.0)
+    for c in range(3):
+        Ct[ns + c] -= u[c] * d1 + rho * dM[c]
+    Ct[ns + 3] -= (C[ns + 3] + P) * d1 / rho + sum(C[ns + c] * dM[c] for c in range(3)) + cpT * dN - dE_extra
+    return Ct
+
+
+@pytest.mark.parametrize("side", range(6))
+def test_nscbc_outflow_closed_forms(side):
+    g, f, om, h = nscbc_case(side)
+    d, minus, bnd = h["d"], h["minus"], h["bnd"]
+    rho, c, P = 1.3, 1.7, 0.9
+    ns = 5
+    f0 = f.copy()
+    # (a) uniform at rest, p = PInf: unchanged
+    O.nscbc_outflow(g, f, side, 0.3, 2.0, P, om)
+    assert np.array_equal(f["Conserved_t"], f0["Conserved_t"])
+    # (b) pressure offset, both K branches; the relaxed wave is LN on a Minus
+    # side (incoming from the far field), L1 on a Plus side
+    for Mmax in (0.3, 1.2):
+        f = f0.copy()
+        un = -0.2 if minus else 0.2                # leaving the domain, subsonic
+        f["velocity"][d] = un
+        PInf, ell = 0.6, 2.0
+        O.nscbc_outflow(g, f, side, Mmax, ell, PInf, om)
+        K = 0.25 * (1 - Mmax * Mmax) * c / ell if Mmax < 0.99 else 0.25 * (c - un * un / c) / ell
+        Lr = K * (P - PInf)
+        d1 = 0.5 * Lr / c**2
+        dM = [0.0, 0.0, 0.0]
+        dM[d] = (Lr / (2 * rho * c)) if minus else (-Lr / (2 * rho * c))
+        f0b = f0.copy()
+        f0b["velocity"][d] = un
+        want = nscbc_expect(f0b, h, ns, d1, dM)
+        np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-13, atol=1e-15)
+    # (c) normal pressure gradient at rest, p = PInf on the boundary: the
+    # outgoing acoustic wave (L1 = -c dp/dn on a Minus side, LN = c dp/dn on a Plus side)
+    f = f0.copy()
+    f["pressure"] = P + (h["along"](0.01) - h["along"](0.01).take([h["bi"]], axis=2 - d))
+    dp_dn = 0.01 * 2.5          # slope along the index direction, times m_d
+    O.nscbc_outflow(g, f, side, 0.3, 2.0, P, om)
+    if minus:
+        L1, LN = -c * dp_dn, 0.0
+    else:
+        L1, LN = 0.0, c * dp_dn
+    d1 = 0.5 * (L1 + LN) / c**2
+    dM = [0.0, 0.0, 0.0]
+    dM[d] = (LN - L1) / (2 * rho * c)
+    want = nscbc_expect(f0, h, ns, d1, dM)
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-12, atol=1e-15)
+    # (d) tangential shear at rest: viscous heating mu (du_t/dn)^2 only
+    t1 = 1 if d == 0 else 0
+    for t in (t1, 3 - d - t1):
+        f = f0.copy()
+        f["velocity"][t] = h["along"](0.03)
+        O.nscbc_outflow(g, f, side, 0.3, 2.0, P, om)
+        dudn = 0.03 * 2.5
+        want = bnd(f0["Conserved_t"]).copy()
+        want[ns + 3] += 2e-3 * dudn * dudn
+        np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-12, atol=1e-15)
+    # (f) normal velocity linear along the normal, at rest on the boundary:
+    # the outgoing acoustic wave of -rho c^2 du/dn, tau_nn = 4/3 mu du/dn the
+    # same at the boundary and the interior point (d tau/dn = 0), viscous
+    # heating d(u tau)/dn = tau du/dn
+    f = f0.copy()
+    a = 0.05
+    f["velocity"][d] = h["along"](a) - h["along"](a).take([h["bi"]], axis=2 - d)
+    O.nscbc_outflow(g, f, side, 0.3, 2.0, P, om)
+    dudn = a * 2.5
+    Lw = rho * c * c * dudn                        # L1 on a Minus side, LN on a Plus side
+    d1 = 0.5 * Lw / c**2
+    dM = [0.0, 0.0, 0.0]
+    dM[d] = (-Lw if minus else Lw) / (2 * rho * c)
+    tau = 2e-3 * 4 * dudn / 3
+    want = nscbc_expect(f0, h, ns, d1, dM, dE_extra=tau * dudn)
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-12, atol=1e-15)
+    # (e) a mass-fraction gradient convected out of the domain: rho u dY/dn
+    # and its enthalpy term; flow entering the domain leaves the species alone
+    for leaving in (True, False):
+        f = f0.copy()
+        un = (-0.3 if minus else 0.3) * (1 if leaving else -1)
+        f["velocity"][d] = un
+        dY = np.array([0.01, -0.004, -0.003, -0.002, -0.001])
+        for s in range(ns):
+            f["MassFracs"][s] = f0["MassFracs"][s] + h["along"](dY[s]) - h["along"](dY[s]).take([h["bi"]], axis=2 - d)
+        O.nscbc_outflow(g, f, side, 0.3, 2.0, P, om)
+        want = bnd(f0["Conserved_t"]).copy()
+        if leaving:
+            T = 1.1
+            Yb = np.array([0.70, 0.20, 0.05, 0.03, 0.02])
+            cpT = O.call("o_GetHeatCapacity", om, T, Yb) * T
+            MixW = O.molar_weight_from_Y(om, Yb)
+            for s in range(ns):
+                LS = un * dY[s] * 2.5
+                W = 1.0 / (1.0 / O.molar_weight_from_Y(om, np.eye(ns)[s]))
+                hs = O.call("o_GetSpeciesEnthalpy", om, s, T)
+                want[s] -= rho * LS
+                want[ns + 3] -= rho * LS * (hs - cpT * MixW / W)
+        np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("side", range(6))
+def test_nscbc_inflow_closed_forms(side):
+    g, f, om, h = nscbc_case(side)
+    d, minus, bnd = h["d"], h["minus"], h["bnd"]
+    rho, c, T = 1.3, 1.7, 1.1
+    ns = 5
+    Yb = np.array([0.70, 0.20, 0.05, 0.03, 0.02])
+    f0 = f.copy()
+    # supersonic inlet: the RHS of the boundary cells is cleared
+    f["velocity"][d] = 1.9 if minus else -1.9
+    O.nscbc_inflow(g, f, side, om)
+    assert np.all(bnd(f["Conserved_t"]) == 0.0)
+    # subsonic, uniform, prescribed dT/dt: rho Y dT/dt / T off the species only
+    f = f0.copy()
+    f["velocity"][d] = 0.4 if minus else -0.4
+    f["dTdtBoundary"] = 0.37
+    O.nscbc_inflow(g, f, side, om)
+    want = bnd(f0["Conserved_t"]).copy()
+    for s in range(ns):
+        want[s] -= (rho * 0.37 / T) * Yb[s]
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-13, atol=1e-15)
+    # subsonic, uniform, prescribed du_n/dt = b: d1 = -/+ (rho b / c + rho c b / (cp T))
+    f = f0.copy()
+    f["velocity"][d] = 0.4 if minus else -0.4
+    b = 0.21
+    f["dudtBoundary"][d] = b
+    O.nscbc_inflow(g, f, side, om)
+    cp = O.call("o_GetHeatCapacity", om, T, Yb)
+    d1 = (rho * b / c + rho * c * b / (cp * T)) * (-1 if minus else 1)
+    want = bnd(f0["Conserved_t"]).copy()
+    for s in range(ns):
+        want[s] -= d1 * Yb[s]
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-12, atol=1e-15)
+    # the momentum and energy rows are never touched by the subsonic inlet
+    assert np.array_equal(bnd(f["Conserved_t"])[ns:], bnd(f0["Conserved_t"])[ns:])
+
+
+def test_max_mach():
+    g, f, om, h = nscbc_case(0)
+    gg = O.Grid(*NSCBC_N)
+    f["velocity"][1] = 0.1
+    f["velocity"][1][2, 3, 4] = -1.02
+    assert O.max_mach(gg, f, 1) == 1.02 / 1.7
+    assert O.max_mach(gg, f, 0) == 0.0
