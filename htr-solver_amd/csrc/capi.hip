@@ -697,7 +697,7 @@ int htr_euler_flux(int dir, int scheme, const htr_grid *g, const htr_fields *f,
 int htr_set_bc(int kind, int side, const htr_grid *g, const htr_fields *f, double Pbc, const htr_mix *m,
                void *stream)
 {
-   if (kind < HTR_BC_DIRICHLET || kind > HTR_BC_ISOTHERMAL_WALL) return set_error(HTR_E_ARG, "bad BC kind %d", kind);
+   if (kind < HTR_BC_DIRICHLET || kind > HTR_BC_NSCBC_OUTFLOW) return set_error(HTR_E_ARG, "bad BC kind %d", kind);
    if (side < 0 || side > 5) return set_error(HTR_E_ARG, "bad BC side %d", side);
    Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
    const int dir = side/2, inward = (side % 2 == 0) ? 1 : -1;
@@ -705,15 +705,11 @@ int htr_set_bc(int kind, int side, const htr_grid *g, const htr_fields *f, doubl
    if ((inward > 0 && g->exec.hi[dir] + 1 > g->bounds.hi[dir]) || (inward < 0 && g->exec.lo[dir] - 1 < g->bounds.lo[dir]))
       return set_error(HTR_E_ARG, "BC: the interior neighbours of the ghost rect are outside the bounds");
    HTR_NEED(MolarFracs); HTR_NEED(velocity); HTR_NEED(temperature); HTR_NEED(pressure);
-   if (kind == HTR_BC_DIRICHLET) { HTR_NEED(MolarFracs_profile); HTR_NEED(velocity_profile); }
-   if (kind != HTR_BC_ADIABATIC_WALL) HTR_NEED(temperature_profile);
-   hipStream_t s = (hipStream_t)stream;
-   switch (kind_of(m)) {
-      case K_CP:  return bc_launch<MixCP>(fr, geo, kind, dir, inward, Pbc, s);
-      case K_AIR: return bc_launch<MixAir>(fr, geo, kind, dir, inward, Pbc, s);
-      case K_30:  return bc_launch<Mix30>(fr, geo, kind, dir, inward, Pbc, s);
-      default: return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
-   }
+   if (kind == HTR_BC_DIRICHLET || kind == HTR_BC_NSCBC_INFLOW) { HTR_NEED(MolarFracs_profile); HTR_NEED(velocity_profile); }
+   if (kind != HTR_BC_ADIABATIC_WALL && kind != HTR_BC_NSCBC_OUTFLOW) HTR_NEED(temperature_profile);
+   if (kind == HTR_BC_NSCBC_INFLOW) { HTR_NEED(SoS); HTR_NEED(Conserved); }
+   if (kind == HTR_BC_NSCBC_OUTFLOW) HTR_NEED(Conserved);
+   HTR_DISPATCH_ALL(bc_launch, fr, m->dev, geo, kind, dir, inward, Pbc, (hipStream_t)stream);
 }
 
 // the interior points of a boundary rect (one cell inward along the normal)
@@ -747,6 +743,20 @@ int htr_nscbc_outflow(int side, const htr_grid *g, const htr_fields *f, double M
    HTR_NEED(temperature); HTR_NEED(Conserved); HTR_NEED(Conserved_t);
    for (int d = 0; d < 3; d++) { HTR_NEED(nType[d]); HTR_NEED(m_d[d]); }
    HTR_DISPATCH_ALL(nscbc_outflow_launch, side/2, side % 2 == 0, fr, m->dev, geo, MaxMach, LengthScale, PInf,
+                    (hipStream_t)stream);
+}
+
+int htr_nscbc_farfield(int side, const htr_grid *g, const htr_fields *f, double PInf, double MaxMach,
+                       double LengthScale, const htr_mix *m, void *stream)
+{
+   Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo); if (rc) return done(rc);
+   if ((rc = nscbc_rect(side, g))) return rc;
+   const int dir = side/2;
+   HTR_NEED(MassFracs); HTR_NEED(pressure); HTR_NEED(velocity); HTR_NEED(rho); HTR_NEED(SoS);
+   HTR_NEED(temperature); HTR_NEED(Conserved); HTR_NEED(Conserved_t);
+   HTR_NEED(MolarFracs_profile); HTR_NEED(velocity_profile); HTR_NEED(temperature_profile);
+   HTR_NEED(nType[dir]); HTR_NEED(m_d[dir]);
+   HTR_DISPATCH_ALL(nscbc_farfield_launch, dir, side % 2 == 0, fr, m->dev, geo, PInf, MaxMach, LengthScale,
                     (hipStream_t)stream);
 }
 

@@ -70,11 +70,13 @@ template<class MX> int nscbc_inflow_launch(int dir, int minus, const htr_fields 
                                            hipStream_t s);
 template<class MX> int nscbc_outflow_launch(int dir, int minus, const htr_fields &f, const DMix *mix, const Geo &g,
                                             double MaxMach, double LengthScale, double PInf, hipStream_t s);
+template<class MX> int nscbc_farfield_launch(int dir, int minus, const htr_fields &f, const DMix *mix, const Geo &g,
+                                             double PInf, double MaxMach, double LengthScale, hipStream_t s);
 int chem_set_launch(int block, int blocks_per_cu);
 int rk_update_launch(const htr_fields &f, const Geo &g, int nEq, int stage, double dt, hipStream_t s);
 int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);
-template<class MX> int bc_launch(const htr_fields &f, const Geo &g, int kind, int dir, int inward, double Pbc,
-                                 hipStream_t s);
+template<class MX> int bc_launch(const htr_fields &f, const DMix *mix, const Geo &g, int kind, int dir, int inward,
+                                 double Pbc, hipStream_t s);
 template<class MX> int rk_stage_launch(const htr_fields &f, const DMix *mix, const Geo &g, int stage, const DtArg &dt,
                                        const htr_view &flags, hipStream_t s);
 int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s);

@@ -378,16 +378,37 @@ int zero_counters(int *counter, hipStream_t s)
 
 // -------------------------------------------------------- boundary conditions
 // Ghost-cell primitives of one boundary (src/prometeo_bc.rg:1159-1265):
-// kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC.  The
-// interior neighbour q of ghost p is one cell along dir, inward (+1 for a
-// Neg side, -1 for a Pos side); the host checks that it lies inside
-// Fluid_bounds.
+// kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC,
+// 3 SetNSCBC_InflowBC (src/prometeo_bc.cc:82-136; kind 4, SetNSCBC_OutflowBC,
+// is the primitive recompute over the rect).  The interior neighbour q of
+// ghost p is one cell along dir, inward (+1 for a Neg side, -1 for a Pos
+// side); the host checks that it lies inside Fluid_bounds.
 template<class MX>
 __global__ void __launch_bounds__(256)
-bc_kernel(const htr_fields f, const Geo g, int kind, int dir, int inward, double Pbc, int64_t n)
+bc_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, int kind, int dir, int inward, double Pbc,
+          int64_t n)
 {
    HTR_POINT_LOOP(g, n)
    constexpr int NS = MX::NS;
+   if (kind == 3) {
+      double X[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) { X[i] = ld<double>(f.MolarFracs_profile, p, i); at<double>(f.MolarFracs, p, i) = X[i]; }
+      const double Tp = ld<double>(f.temperature_profile, p);
+      at<double>(f.temperature, p) = Tp;
+#pragma unroll
+      for (int i = 0; i < 3; i++) at<double>(f.velocity, p, i) = ld<double>(f.velocity_profile, p, i);
+      if (fabs(ld<double>(f.velocity_profile, p, dir)) >= ld<double>(f.SoS, p)) {
+         at<double>(f.pressure, p) = Pbc;           // supersonic: everything imposed
+      } else {
+         // pressure of the transported density at the imposed T and X
+         double rho = 0.0;
+#pragma unroll
+         for (int i = 0; i < NS; i++) rho += ld<double>(f.Conserved, p, i);
+         at<double>(f.pressure, p) = MX::PFromRhoAndT(*mix, rho, MX::MolarWeightFromX(*mix, X), Tp);
+      }
+      return;
+   }
    P3 q = p;
    setc(g, q, dir, p.c[dir] + inward);
    if (kind == 0) {
@@ -457,8 +478,12 @@ int rk_init_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
 { HTR_LAUNCH_POINT(rk_init_kernel, f, g, nEq) }
 
 template<class MX>
-int bc_launch(const htr_fields &f, const Geo &g, int kind, int dir, int inward, double Pbc, hipStream_t s)
-{ HTR_LAUNCH_POINT(bc_kernel<MX>, f, g, kind, dir, inward, Pbc) }
+int bc_launch(const htr_fields &f, const DMix *mix, const Geo &g, int kind, int dir, int inward, double Pbc,
+              hipStream_t s)
+{
+   if (kind == 4) return primitive_launch<MX>(f, mix, g, s);
+   HTR_LAUNCH_POINT(bc_kernel<MX>, f, mix, g, kind, dir, inward, Pbc)
+}
 
 int zero_rhs_launch(const htr_fields &f, const Geo &g, int nEq, hipStream_t s)
 { HTR_LAUNCH_POINT(zero_rhs_kernel, f, g, nEq) }
@@ -565,7 +590,7 @@ int spectral_radius_noalloc_launch(const htr_fields &f, const DMix *mix, const G
                                                    hipStream_t);                                                   \
    template int rk_stage_launch<MX>(const htr_fields &, const DMix *, const Geo &, int, const DtArg &,           \
                                     const htr_view &, hipStream_t);                                               \
-   template int bc_launch<MX>(const htr_fields &, const Geo &, int, int, int, double, hipStream_t);
+   template int bc_launch<MX>(const htr_fields &, const DMix *, const Geo &, int, int, int, double, hipStream_t);
 HTR_INST_POINT(MixCP)
 HTR_INST_POINT(MixAir)
 HTR_INST_POINT(Mix30)

@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("HTR_AMD_LIBRARY") or os.path.join(PKG, "libhtr_amd.so
 
 HTR_EULER_HYBRID, HTR_EULER_TENO, HTR_EULER_TENOA, HTR_EULER_TENOLAD, HTR_EULER_KG = range(5)
 HTR_CHEM_FP64, HTR_CHEM_FP32_IR = 0, 1
-HTR_BC_DIRICHLET, HTR_BC_ADIABATIC_WALL, HTR_BC_ISOTHERMAL_WALL = 0, 1, 2
+HTR_BC_DIRICHLET, HTR_BC_ADIABATIC_WALL, HTR_BC_ISOTHERMAL_WALL, HTR_BC_NSCBC_INFLOW, HTR_BC_NSCBC_OUTFLOW = 0, 1, 2, 3, 4
 ERRORS = {-1: "HTR_E_ARG", -2: "HTR_E_UNSUPPORTED", -3: "HTR_E_DEVICE", -4: "HTR_E_FIELD", -5: "HTR_E_NUMERIC"}
 
 
@@ -61,7 +61,8 @@ EXPORTS = [
     "htr_rk_update", "htr_rk_init", "htr_full_rhs",
     "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
     "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl", "htr_set_bc",
-    "htr_max_spectral_radius_noalloc", "htr_nscbc_inflow", "htr_nscbc_outflow", "htr_max_mach",
+    "htr_max_spectral_radius_noalloc", "htr_nscbc_inflow", "htr_nscbc_outflow", "htr_nscbc_farfield",
+    "htr_max_mach",
 ]
 
 _lib = None
@@ -95,6 +96,7 @@ def _declare(L):
         "htr_nscbc_inflow": (i, [i, G, F, vp, vp]),
         "htr_nscbc_outflow": (i, [i, G, F, d, d, d, vp, vp]),
         "htr_max_mach": (i, [i, G, F, dp, vp]),
+        "htr_nscbc_farfield": (i, [i, G, F, d, d, d, vp, vp]),
         "htr_mix_create_constprop": (i, [C.POINTER(vp), d, d, d, i, d, d, d]),
         "htr_mix_create_airmix": (i, [C.POINTER(vp), d, d, d, i, ip, dp]),
         "htr_mix_create_species": (i, [C.POINTER(vp), i, dp, fp, ip, dp, dp, dp, ip, dp]),

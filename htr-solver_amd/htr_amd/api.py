@@ -24,6 +24,7 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "rk_update", "rk_init", "full_rhs", "stream_ptr", "profile", "profile_report",
            "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve", "set_bc",
            "BC_SIDES", "update_using_flux_nscbc_inflow", "update_using_flux_nscbc_outflow",
+           "update_using_flux_nscbc_far_field",
            "calculate_max_mach_number"]
 
 
@@ -342,6 +343,13 @@ def update_using_flux_nscbc_outflow(fs, side, mix, MaxMach, LengthScale, PInf, g
     towards PInf over LengthScale; MaxMach from calculate_max_mach_number."""
     check(lib().htr_nscbc_outflow(_side(side), C.byref(grid), C.byref(fs.c()), float(MaxMach), float(LengthScale),
                                   float(PInf), mix.h, stream_ptr(stream)))
+
+
+def update_using_flux_nscbc_far_field(fs, side, mix, PInf, MaxMach, LengthScale, grid, stream=None):
+    """UpdateUsingFluxNSCBCFarField{Minus,Plus}Side: LODI corrections of
+    Conserved_t towards the far-field state of the profile fields."""
+    check(lib().htr_nscbc_farfield(_side(side), C.byref(grid), C.byref(fs.c()), float(PInf), float(MaxMach),
+                                   float(LengthScale), mix.h, stream_ptr(stream)))
 
 
 def calculate_max_mach_number(fs, direction, grid=None, stream=None):

@@ -189,6 +189,9 @@ int htr_shock_sensor(int dir, const htr_grid *g, const htr_fields *f, const htr_
 #define HTR_BC_DIRICHLET       0   /* SetDirichletBC: X, u, T from the profiles, p = Pbc */
 #define HTR_BC_ADIABATIC_WALL  1   /* SetAdiabaticWallBC: X, p, T of the interior neighbour, u = 0 */
 #define HTR_BC_ISOTHERMAL_WALL 2   /* SetIsothermalWallBC: X, p of the neighbour, T from the profile, u = 0 */
+#define HTR_BC_NSCBC_INFLOW    3   /* SetNSCBC_InflowBC (src/prometeo_bc.cu:163-204): X, u, T from the profiles,
+                                      p = Pbc if |u_n| >= SoS, else p(rho of Conserved, T, X) */
+#define HTR_BC_NSCBC_OUTFLOW   4   /* SetNSCBC_OutflowBC (src/prometeo_bc.cu:256-290): primitives from Conserved */
 /* Primitive variables of the ghost cells of one boundary, the BC part of
  * UpdateGhostPrimitives: g->exec is the ghost rect of the boundary, side
  * 0..5 = xNeg, xPos, yNeg, yPos, zNeg, zPos names the boundary (the interior
@@ -218,6 +221,13 @@ int htr_nscbc_inflow(int side, const htr_grid *g, const htr_fields *f, const htr
  *   temperature, Conserved. */
 int htr_nscbc_outflow(int side, const htr_grid *g, const htr_fields *f, double MaxMach, double LengthScale,
                       double PInf, const htr_mix *m, void *stream);
+/* UpdateUsingFluxNSCBCFarField{Minus,Plus}Side<dir>::gpu_base_impl
+ *   (src/prometeo_rhs.cu:1225-1293 and its Plus twin): far-field state
+ *   TInf, vInf, XiInf = temperature_profile, velocity_profile,
+ *   MolarFracs_profile of each boundary cell; PInf, MaxMach, LengthScale as
+ *   for the outflow. */
+int htr_nscbc_farfield(int side, const htr_grid *g, const htr_fields *f, double PInf, double MaxMach,
+                       double LengthScale, const htr_mix *m, void *stream);
 /* CalculateMaxMachNumber (src/prometeo_stat.rg:223-234): max |u_dir|/SoS over
  * g->exec into *out; waits for the stream.  An empty rect gives 0 (the
  * reference starts from -inf, but its tiles always hold cells). */

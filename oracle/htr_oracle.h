@@ -213,7 +213,9 @@ void   o_chem_source(const o_grid *g, const o_fields *f, const o_mix *m);
 void   o_chem_update(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps);
 double o_max_spectral_radius(const o_grid *g, const o_fields *f, const o_mix *m);
 /* ghost-cell primitives of one boundary (src/prometeo_bc.rg:1159-1265):
- * kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC; side
+ * kind 0 SetDirichletBC, 1 SetAdiabaticWallBC, 2 SetIsothermalWallBC,
+ * 3 SetNSCBC_InflowBC (src/prometeo_bc.cc:82-136), 4 SetNSCBC_OutflowBC
+ * (src/prometeo_bc.cc:156-189: primitives recomputed from Conserved); side
  * 0..5 = xNeg, xPos, yNeg, yPos, zNeg, zPos; the rect of g = the ghost cells */
 void   o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc, const o_mix *m);
 /* NSCBC flux corrections of the boundary cells of one side (the rect of g,
@@ -223,6 +225,11 @@ void   o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double P
 void   o_nscbc_inflow(const o_grid *g, const o_fields *f, int side, const o_mix *m);
 void   o_nscbc_outflow(const o_grid *g, const o_fields *f, int side, double MaxMach, double LengthScale,
                        double PInf, const o_mix *m);
+/* UpdateUsingFluxNSCBCFarField{Minus,Plus}Side (src/prometeo_rhs.inl:1633-1931):
+ * the far-field state is TInf = temperature_profile, vInf = velocity_profile,
+ * XiInf = MolarFracs_profile at each boundary cell */
+void   o_nscbc_farfield(const o_grid *g, const o_fields *f, int side, double PInf, double MaxMach,
+                        double LengthScale, const o_mix *m);
 /* max |u_dir|/SoS over the rect of g (CalculateMaxMachNumber, src/prometeo_stat.rg:223-234) */
 double o_max_mach(const o_grid *g, const o_fields *f, int dir);
 /* SSP-RK3 update of Conserved (src/prometeo_rk.rg:88-112) */

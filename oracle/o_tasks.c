@@ -598,7 +598,8 @@ void o_full_rhs(const o_grid *g, const o_fields *f, int scheme, double RK0, doub
 
 /* ------------------------------------------------------------------------
  * Boundary ghost cells (src/prometeo_bc.rg:1159-1265): SetDirichletBC,
- * SetAdiabaticWallBC, SetIsothermalWallBC.  The interior neighbour of a
+ * SetAdiabaticWallBC, SetIsothermalWallBC; SetNSCBC_InflowBC and
+ * SetNSCBC_OutflowBC (src/prometeo_bc.cc:82-189).  The interior neighbour of a
  * ghost is c_int = c + 1 along the normal for a Neg side, c - 1 for a Pos
  * side (mk_cint of the reference).
  * Pinned by tests/test_oracle_pins.py (wall and Dirichlet ghost values,
@@ -606,6 +607,10 @@ void o_full_rhs(const o_grid *g, const o_fields *f, int scheme, double RK0, doub
  */
 void o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc, const o_mix *m)
 {
+   if (kind == 4) {                    /* SetNSCBC_OutflowBC: UpdatePrimitive over the rect */
+      o_update_primitive(g, f, m);
+      return;
+   }
    const int ns = m->nSpec;
    const int dir = side/2, inward = (side % 2 == 0) ? 1 : -1;
    const long N = (long)g->nx*g->ny*g->nz;
@@ -616,6 +621,22 @@ void o_set_bc(const o_grid *g, const o_fields *f, int kind, int side, double Pbc
             const long p = (long)i + (long)g->nx*((long)j + (long)g->ny*(long)k);
             c[dir] += inward;
             const long q = (long)c[0] + (long)g->nx*((long)c[1] + (long)g->ny*(long)c[2]);
+            if (kind == 3) {
+               for (int s = 0; s < ns; s++) f->MolarFracs[s*N + p] = f->MolarFracs_profile[s*N + p];
+               f->temperature[p] = f->temperature_profile[p];
+               for (int s = 0; s < 3; s++) f->velocity[s*N + p] = f->velocity_profile[s*N + p];
+               if (fabs(f->velocity_profile[dir*N + p]) >= f->SoS[p]) {
+                  f->pressure[p] = Pbc;        /* supersonic: everything imposed */
+               } else {
+                  /* pressure from the transported density and the imposed T and X */
+                  double rho = 0.0, X[O_NSP_MAX];
+                  if (m->constprop) rho = f->Conserved[p];
+                  else for (int s = 0; s < ns; s++) rho += f->Conserved[s*N + p];
+                  for (int s = 0; s < ns; s++) X[s] = f->MolarFracs_profile[s*N + p];
+                  f->pressure[p] = o_GetPFromRhoAndT(m, rho, o_GetMolarWeightFromXi(m, X), f->temperature_profile[p]);
+               }
+               continue;
+            }
             if (kind == 0) {
                for (int s = 0; s < ns; s++) f->MolarFracs[s*N + p] = f->MolarFracs_profile[s*N + p];
                for (int s = 0; s < 3; s++) f->velocity[s*N + p] = f->velocity_profile[s*N + p];
@@ -823,6 +844,142 @@ void o_nscbc_outflow(const o_grid *g, const o_fields *f, int side, double MaxMac
             for (int s = 0; s < ns; s++) {
                const double dEdYi = o_GetSpeciesEnthalpy(m, s, T) - cpT_Wmix/species_W(m, s);
                Ct[irE*N + p] -= (rho*LS[s]*dEdYi);
+            }
+         }
+}
+
+/* RHS update of the far-field LODI fluxes (no viscous terms),
+ * src/prometeo_rhs.inl:1713-1735, 1759-1779 */
+static void lodi_update(const o_fields *f, long N, long p, const o_mix *m, int iN, int iT1, int iT2,
+                        const double *Y, const double *LS, const double *u, double rho, double P,
+                        double T, double cpT, double MixW, double d1, const double *dM, double dN)
+{
+   const int ns = m->nSpec, irU = ns, irE = ns + 3;
+   double *Ct = f->Conserved_t;
+   const double *Cs = f->Conserved;
+   for (int s = 0; s < ns; s++) Ct[s*N + p] -= (d1*Y[s] + rho*LS[s]);
+   Ct[(irU+iN)*N + p]  -= (u[iN]*d1 + rho*dM[iN]);
+   Ct[(irU+iT1)*N + p] -= (u[iT1]*d1 + rho*dM[iT1]);
+   Ct[(irU+iT2)*N + p] -= (u[iT2]*d1 + rho*dM[iT2]);
+   Ct[irE*N + p] -= ((Cs[irE*N + p] + P)*d1/rho
+                     + Cs[(irU+iN)*N + p]*dM[iN]
+                     + Cs[(irU+iT1)*N + p]*dM[iT1]
+                     + Cs[(irU+iT2)*N + p]*dM[iT2]
+                     + cpT*dN);
+   const double cpT_MixW = cpT*MixW;
+   for (int s = 0; s < ns; s++) {
+      const double dEdYi = o_GetSpeciesEnthalpy(m, s, T) - cpT_MixW/species_W(m, s);
+      Ct[irE*N + p] -= (rho*LS[s]*dEdYi);
+   }
+}
+
+void o_nscbc_farfield(const o_grid *g, const o_fields *f, int side, double PInf, double MaxMach,
+                      double LengthScale, const o_mix *m)
+{
+   const int ns = m->nSpec;
+   const int dir = side/2, minus = (side % 2 == 0);
+   const int iN = dir, iT1 = (dir == 0) ? 1 : 0, iT2 = (dir == 2) ? 1 : 2;
+   const int irU = ns, irE = ns + 3;
+   const long N = NN(g);
+   for (int k = g->lo[2]; k <= g->hi[2]; k++)
+      for (int j = g->lo[1]; j <= g->hi[1]; j++)
+         for (int i = g->lo[0]; i <= g->hi[0]; i++) {
+            int ci[3] = { i, j, k };
+            const long p = L(g, i, j, k);
+            ci[dir] += minus ? 1 : -1;
+            const long q = L(g, ci[0], ci[1], ci[2]);
+            const int t = f->nType[iN][p];
+            const double md = f->m_d[iN][p];
+            const double SoS = f->SoS[p], T = f->temperature[p], rho = f->rho[p], P = f->pressure[p];
+            double u[3], vInf[3], Y[O_NSP_MAX], LS[O_NSP_MAX];
+            for (int c = 0; c < 3; c++) { u[c] = f->velocity[c*N + p]; vInf[c] = f->velocity_profile[c*N + p]; }
+            for (int s = 0; s < ns; s++) Y[s] = f->MassFracs[s*N + p];
+            const double TInf = f->temperature_profile[p];
+            const double un = u[iN];
+            /* characteristic velocities (entropy and species are not extrapolated) */
+            const double lambda_1 = un - SoS;
+            const double lambda   = un;
+            const double lambda_N = un + SoS;
+            double dp_dn, dun_dn;
+            if (minus) {
+               dp_dn  = deriv_left(t, P, f->pressure[q], md);
+               dun_dn = deriv_left(t, un, f->velocity[iN*N + q], md);
+            } else {
+               dp_dn  = deriv_right(t, f->pressure[q], P, md);
+               dun_dn = deriv_right(t, f->velocity[iN*N + q], un, md);
+            }
+            const double MixW = o_GetMolarWeightFromYi(m, Y);
+            const double cpT = o_GetHeatCapacity(m, T, Y)*T;
+            const double MaInflow = minus ? dmax(lambda/SoS, 0.0) : dmax(-lambda/SoS, 0.0);
+            const double sigma = 0.25;
+            double *Ct = f->Conserved_t;
+            const double *Cs = f->Conserved;
+            if (MaInflow > 0) {
+               const double coeff = 39;
+               const double Kp = sigma*(1 + coeff*MaInflow*MaInflow)*SoS/LengthScale;
+               if (MaInflow > 1) {
+                  /* supersonic inflow: relax every conserved variable to the far field */
+                  const double rhoTarget = o_GetRho(m, PInf, TInf, MixW);
+                  for (int s = 0; s < ns; s++)
+                     Ct[s*N + p] += Kp*(rhoTarget*f->MolarFracs_profile[s*N + p]*species_W(m, s)/MixW - Cs[s*N + p]);
+                  for (int c = 0; c < 3; c++)
+                     Ct[(irU+c)*N + p] += Kp*(rhoTarget*vInf[c] - Cs[(irU+c)*N + p]);
+                  double v2 = 0.0;
+                  v2 += vInf[0]*vInf[0]; v2 += vInf[1]*vInf[1]; v2 += vInf[2]*vInf[2];
+                  Ct[irE*N + p] += Kp*(rhoTarget*(0.5*v2 + o_GetInternalEnergy(m, TInf, Y)) - Cs[irE*N + p]);
+               } else {
+                  /* subsonic inflow */
+                  const double K = sigma*(coeff + 1)*MaInflow*MaInflow*SoS/LengthScale;
+                  double L1, LN;
+                  if (minus) {
+                     L1 = lambda_1*(dp_dn - rho*SoS*dun_dn);
+                     LN = Kp*(P - PInf);
+                  } else {
+                     L1 = Kp*(P - PInf);
+                     LN = lambda_N*(dp_dn + rho*SoS*dun_dn);
+                  }
+                  for (int s = 0; s < ns; s++)
+                     LS[s] = K*(Y[s] - f->MolarFracs_profile[s*N + p]*species_W(m, s)/MixW);
+                  double L2 = K*(TInf/T - 1) + (LN + L1)/(2*rho*cpT);
+                  for (int s = 0; s < ns; s++) L2 -= MixW/species_W(m, s)*LS[s];
+                  L2 *= -rho*SoS*SoS;
+                  const double d1 = (0.5*(L1 + LN) - L2)/(SoS*SoS);
+                  double dM[3];
+                  dM[iN]  = (LN - L1)/(2*rho*SoS);
+                  dM[iT1] = K*(u[iT1] - vInf[iT1]);
+                  dM[iT2] = K*(u[iT2] - vInf[iT2]);
+                  const double dN = L2/(SoS*SoS);
+                  lodi_update(f, N, p, m, iN, iT1, iT2, Y, LS, u, rho, P, T, cpT, MixW, d1, dM, dN);
+               }
+            } else {
+               /* outflow */
+               const double Kout = (MaxMach < 0.99) ? sigma*(1.0 - MaxMach*MaxMach)*SoS/LengthScale
+                                                    : sigma*(SoS - (un*un)/SoS)/LengthScale;
+               double L1, L2, LN, duT1, duT2;
+               if (minus) {
+                  L1 = lambda_1*(dp_dn - rho*SoS*dun_dn);
+                  L2 = lambda*(dp_dn - SoS*SoS*deriv_left(t, rho, f->rho[q], md));
+                  for (int s = 0; s < ns; s++)
+                     LS[s] = lambda*deriv_left(t, Y[s], f->MassFracs[s*N + q], md);
+                  LN = (lambda_N < 0) ? lambda_N*(dp_dn + rho*SoS*dun_dn) : Kout*(P - PInf);
+                  duT1 = deriv_left(t, u[iT1], f->velocity[iT1*N + q], md);
+                  duT2 = deriv_left(t, u[iT2], f->velocity[iT2*N + q], md);
+               } else {
+                  L1 = (lambda_1 > 0) ? lambda_1*(dp_dn - rho*SoS*dun_dn) : Kout*(P - PInf);
+                  L2 = lambda*(dp_dn - SoS*SoS*deriv_right(t, f->rho[q], rho, md));
+                  for (int s = 0; s < ns; s++)
+                     LS[s] = lambda*deriv_right(t, f->MassFracs[s*N + q], Y[s], md);
+                  LN = lambda_N*(dp_dn + rho*SoS*dun_dn);
+                  duT1 = deriv_right(t, f->velocity[iT1*N + q], u[iT1], md);
+                  duT2 = deriv_right(t, f->velocity[iT2*N + q], u[iT2], md);
+               }
+               const double d1 = (0.5*(L1 + LN) - L2)/(SoS*SoS);
+               double dM[3];
+               dM[iN]  = (LN - L1)/(2*rho*SoS);
+               dM[iT1] = lambda*duT1;
+               dM[iT2] = lambda*duT2;
+               const double dN = L2/(SoS*SoS);
+               lodi_update(f, N, p, m, iN, iT1, iT2, Y, LS, u, rho, P, T, cpT, MixW, d1, dM, dN);
             }
          }
 }

@@ -125,6 +125,7 @@ def _declare(L):
         "o_nscbc_inflow": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, vp]),
         "o_nscbc_outflow": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, d, d, d, vp]),
         "o_max_mach": (d, [C.POINTER(OGrid), C.POINTER(OFields), i]),
+        "o_nscbc_farfield": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, d, d, d, vp]),
         "o_full_rhs": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, d, d, d, i, vp]),
         "o_lu_solve": (None, [i, _dp, _dp]),
         "o_rosenbrock_testprob": (None, [_dp, d, d, C.POINTER(C.c_int32)]),
@@ -404,6 +405,12 @@ def nscbc_inflow(g, f, side, m):
 def nscbc_outflow(g, f, side, MaxMach, LengthScale, PInf, m):
     """UpdateUsingFluxNSCBCOutflow{Minus,Plus}Side over the rect of g (side 0..5)."""
     lib().o_nscbc_outflow(C.byref(g.c()), C.byref(f.c()), side, MaxMach, LengthScale, PInf, m.ptr)
+
+
+def nscbc_farfield(g, f, side, PInf, MaxMach, LengthScale, m):
+    """UpdateUsingFluxNSCBCFarField{Minus,Plus}Side over the rect of g (side 0..5);
+    the far-field state comes from the profile fields."""
+    lib().o_nscbc_farfield(C.byref(g.c()), C.byref(f.c()), side, PInf, MaxMach, LengthScale, m.ptr)
 
 
 def max_mach(g, f, d):

@@ -194,7 +194,7 @@ def main():
             d, neg = side // 2, side % 2 == 0
             lo, hi = [0, 0, 0], [7, 7, 7]
             lo[d] = hi[d] = 0 if neg else 7
-            for kind in (0, 1, 2):
+            for kind in (0, 1, 2, 3, 4):
                 g, f = oracle_fields(dict(st, **prof), 5, n)
                 gr = g.with_rect(tuple(lo), tuple(hi))
                 O.set_bc(gr, f, kind, side, 0.9, om)
@@ -219,16 +219,20 @@ def main():
             fs = to_host_fs(sn, 5, nn, profiles=True)
             mm = O.max_mach(g, f, d)
             report("max_mach_d%d_side%d" % (d, side), H.calculate_max_mach_number(fs, d), mm, 216)
-            for kind, Mmax in (("inflow", None), ("outflow", 0.5), ("outflow_sup", max(mm, 1.0))):
+            for kind, Mmax in (("inflow", None), ("outflow", 0.5), ("outflow_sup", max(mm, 1.0)),
+                               ("farfield", -0.5), ("farfield_sup", -max(mm, 1.0))):
                 f["Conserved_t"] = 0.0
                 fs["Conserved_t"] = 0.0
                 gr, grid = g.with_rect(tuple(lo), tuple(hi)), fs.grid(tuple(lo), tuple(hi))
                 if Mmax is None:
                     O.nscbc_inflow(gr, f, side, omn)
                     H.update_using_flux_nscbc_inflow(fs, side, gmn, grid)
-                else:
+                elif Mmax > 0:
                     O.nscbc_outflow(gr, f, side, Mmax, 0.8, 0.95, omn)
                     H.update_using_flux_nscbc_outflow(fs, side, gmn, Mmax, 0.8, 0.95, grid)
+                else:
+                    O.nscbc_farfield(gr, f, side, 0.95, -Mmax, 0.8, omn)
+                    H.update_using_flux_nscbc_far_field(fs, side, gmn, 0.95, -Mmax, 0.8, grid)
                 report("nscbc_%s_side%d" % (kind, side), fs["Conserved_t"].numpy(), f["Conserved_t"], 36)
     print("DONE %.1f s" % (time.time() - t0), flush=True)
 

@@ -144,8 +144,9 @@ def wall_state(nx, ny, nz, seed=3, collocated=False):
 def nscbc_state(n, seed=5):
     """Air box bounded by collocated boundary nodes (L_C .. R_C) in all three
     directions on stretched grids, with a mean flow through it, a few
-    supersonic cells on every face and random inflow-profile time
-    derivatives (dudtBoundary, dTdtBoundary): the inputs of the NSCBC tasks."""
+    supersonic cells on every face, random inflow-profile time derivatives
+    (dudtBoundary, dTdtBoundary) and a far-field state in the profile fields:
+    the inputs of the NSCBC tasks."""
     st, meta = cases.air_box(n, seed, composition="per-cell")
     rng = np.random.default_rng(seed)
     dims = [cases.wall_metrics(n - 2, width=1.0, stretching=1.2 + 0.1 * d, collocated=True)[:4] for d in range(3)]
@@ -162,4 +163,10 @@ def nscbc_state(n, seed=5):
     st["velocity"] = u
     st["dudtBoundary"] = rng.normal(size=(3, n, n, n)) * scale
     st["dTdtBoundary"] = rng.normal(size=(n, n, n)) * 0.1 * np.mean(st["temperature"])
+    # far-field state: a perturbed copy of the mean flow
+    X = st["MolarFracs"]
+    st["MolarFracs_profile"] = (X.mean(axis=(1, 2, 3))[:, None, None, None] * (1 + 0.1 * rng.random(X.shape)))
+    st["MolarFracs_profile"] /= st["MolarFracs_profile"].sum(axis=0)
+    st["velocity_profile"] = u * (1 + 0.05 * rng.normal(size=u.shape))
+    st["temperature_profile"] = st["temperature"] * (1 + 0.05 * rng.random((n, n, n)))
     return st, meta, (n, n, n)

@@ -45,6 +45,14 @@ a swapped stencil row, a wrong blend -- fails it:
                      gives rho u dY/dn and its enthalpy term, and nothing when
                      the flow enters; a supersonic inlet clears the RHS;
                      prescribed dT/dt and du_n/dt give the inlet waves.
+                     Far field: a cell at the far-field state is left alone
+                     (supersonic and subsonic inflow); a temperature or
+                     pressure offset from it gives the relaxation waves
+                     K (TInf/T - 1) and Kp (p - PInf); an outflow at PInf
+                     is left alone.  Ghost values: the NSCBC inflow ghost
+                     recovers the pressure of the transported density at the
+                     imposed T and X (or takes Pbc when supersonic); the
+                     outflow ghost is the primitive recompute.
 """
 import numpy as np
 import pytest
@@ -558,3 +566,119 @@ def test_max_mach():
     f["velocity"][1][2, 3, 4] = -1.02
     assert O.max_mach(gg, f, 1) == 1.02 / 1.7
     assert O.max_mach(gg, f, 0) == 0.0
+
+
+def far_field_state(f, om, PInf, TInf, Xinf, vInf):
+    """Fills the cells of f with the far-field state itself (primitive,
+    properties used by the far-field fluxes, conserved)."""
+    MixW = O.molar_weight_from_X(om, Xinf)
+    Y = O.mass_fractions(om, MixW, Xinf)
+    rho = O.call("o_GetRho", om, PInf, TInf, MixW)
+    e = O.call("o_GetInternalEnergy", om, TInf, Y)
+    f["MassFracs"] = Y[:, None, None, None]
+    f["MolarFracs_profile"] = np.asarray(Xinf)[:, None, None, None]
+    f["temperature_profile"] = TInf
+    f["velocity_profile"] = np.asarray(vInf)[:, None, None, None]
+    f["temperature"], f["pressure"], f["rho"] = TInf, PInf, rho
+    f["velocity"] = np.asarray(vInf)[:, None, None, None]
+    C = np.concatenate([rho * Y, rho * np.asarray(vInf), [rho * (e + 0.5 * float(np.dot(vInf, vInf)))]])
+    f["Conserved"] = C[:, None, None, None]
+    return rho, Y
+
+
+@pytest.mark.parametrize("side", range(6))
+def test_nscbc_far_field_closed_forms(side):
+    g, f, om, h = nscbc_case(side)
+    d, minus, bnd = h["d"], h["minus"], h["bnd"]
+    ns, c, ell = 5, 1.7, 0.9
+    PInf, TInf = 0.9, 1.1
+    Xinf = np.array([0.75, 0.2, 0.03, 0.01, 0.01])
+    sgn = 1 if minus else -1                       # + normal velocity enters on a Minus side
+    for Ma in (1.4, 0.35):                         # supersonic, subsonic inflow
+        vInf = np.array([0.1, -0.05, 0.07])
+        vInf[d] = sgn * Ma * c
+        rho, Y = far_field_state(f, om, PInf, TInf, Xinf, vInf)
+        f0 = f.copy()
+        O.nscbc_farfield(g, f, side, PInf, 0.3, ell, om)
+        scale = np.abs(bnd(f0["Conserved"])).max(axis=(1, 2))[:, None, None] * c / ell
+        err = np.abs(bnd(f["Conserved_t"]) - bnd(f0["Conserved_t"])) / scale
+        assert err.max() < 1e-13, (Ma, err.max())
+    # subsonic inflow, T off the far-field value: d1 = rho K (TInf/T - 1),
+    # dN = -d1, K = 0.25 * 40 Ma^2 c / L
+    Ma = 0.35
+    vInf = np.array([0.1, -0.05, 0.07])
+    vInf[d] = sgn * Ma * c
+    rho, Y = far_field_state(f, om, PInf, TInf, Xinf, vInf)
+    T = 1.25
+    f["temperature"] = T
+    f0 = f.copy()
+    O.nscbc_farfield(g, f, side, PInf, 0.3, ell, om)
+    K = 0.25 * 40 * Ma * Ma * c / ell
+    d1 = rho * K * (TInf / T - 1)
+    cpT = O.call("o_GetHeatCapacity", om, T, Y) * T
+    want = nscbc_expect(f0, h, ns, d1, [0.0, 0.0, 0.0], dN=-d1, cpT=cpT)
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-11, atol=1e-12)
+    # subsonic inflow, p off PInf: the relaxed wave is LN on a Minus side, L1
+    # on a Plus side, Kp (p - PInf) with Kp = 0.25 (1 + 39 Ma^2) c / L
+    rho, Y = far_field_state(f, om, PInf, TInf, Xinf, vInf)
+    P = 1.05
+    f["pressure"] = P
+    f0 = f.copy()
+    O.nscbc_farfield(g, f, side, PInf, 0.3, ell, om)
+    Kp = 0.25 * (1 + 39 * Ma * Ma) * c / ell
+    Lr = Kp * (P - PInf)
+    cpT = O.call("o_GetHeatCapacity", om, TInf, Y) * TInf
+    L2 = -rho * c * c * (Lr / (2 * rho * cpT))
+    d1 = (0.5 * Lr - L2) / c**2
+    dM = [0.0, 0.0, 0.0]
+    dM[d] = (Lr if minus else -Lr) / (2 * rho * c)
+    want = nscbc_expect(f0, h, ns, d1, dM, dN=L2 / c**2, cpT=cpT)
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-11, atol=1e-12)
+    # outflow at PInf, uniform: left alone; with p off PInf: relaxation as in
+    # the outflow task (no viscous terms)
+    vInf[d] = -sgn * 0.3 * c
+    rho, Y = far_field_state(f, om, PInf, TInf, Xinf, vInf)
+    f0 = f.copy()
+    O.nscbc_farfield(g, f, side, PInf, 0.3, ell, om)
+    assert np.array_equal(f["Conserved_t"], f0["Conserved_t"])
+    f["pressure"] = P
+    f0 = f.copy()
+    O.nscbc_farfield(g, f, side, PInf, 0.3, ell, om)
+    Lr = 0.25 * (1 - 0.09) * c / ell * (P - PInf)
+    dM = [0.0, 0.0, 0.0]
+    dM[d] = (Lr if minus else -Lr) / (2 * rho * c)
+    want = nscbc_expect(f0, h, ns, 0.5 * Lr / c**2, dM)
+    np.testing.assert_allclose(bnd(f["Conserved_t"]), want, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("side", [0, 3, 5])
+def test_nscbc_ghost_values(side):
+    """SetNSCBC_InflowBC: X, T, u from the profiles; p from the transported
+    density at the imposed T and X (the pressure the density was made from),
+    Pbc where the imposed normal velocity is supersonic.  SetNSCBC_OutflowBC:
+    the primitives of the transported conserved state."""
+    g, f, om, h = nscbc_case(side)
+    d, bnd = h["d"], h["bnd"]
+    PInf, TInf, c = 0.9, 1.1, 1.7
+    Xinf = np.array([0.75, 0.2, 0.03, 0.01, 0.01])
+    vInf = np.array([0.3, -0.2, 0.1])
+    far_field_state(f, om, PInf, TInf, Xinf, vInf)
+    f["SoS"] = c
+    f["pressure"] = 0.0
+    O.set_bc(g, f, 3, side, 7.0, om)
+    np.testing.assert_allclose(bnd(f["pressure"]), PInf, rtol=1e-14)
+    assert np.all(bnd(f["temperature"]) == TInf)
+    f["velocity_profile"][d] = 1.01 * c
+    O.set_bc(g, f, 3, side, 7.0, om)
+    assert np.all(bnd(f["pressure"]) == 7.0)
+    assert np.all(bnd(f["velocity"])[d] == 1.01 * c)
+    # outflow ghost: primitives of the conserved state (temperature guess off)
+    far_field_state(f, om, PInf, TInf, Xinf, vInf)
+    f["temperature"] = 1.3
+    f["pressure"] = 0.0
+    f["velocity"] = 0.0
+    O.set_bc(g, f, 4, side, 0.0, om)
+    np.testing.assert_allclose(bnd(f["pressure"]), PInf, rtol=1e-10)
+    np.testing.assert_allclose(bnd(f["temperature"]), TInf, rtol=1e-10)
+    np.testing.assert_allclose(bnd(f["velocity"]), np.broadcast_to(vInf[:, None, None], bnd(f["velocity"]).shape),
+                               rtol=1e-13)
