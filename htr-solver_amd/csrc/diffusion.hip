@@ -6,6 +6,7 @@
 // line segments walked face by face, the divergence fused (below).
 #include "common.hpp"
 #include "kernels.hpp"
+#include <type_traits>
 
 namespace htr {
 
@@ -47,52 +48,91 @@ struct Diffusion {
       for (int i = 0; i < 3; i++) { g1[0][i] = g1[1][i]; g2[0][i] = g2[1][i]; }
    }
 
-   // GetDiffusionFlux (rhs.inl:1140-1218) at the face between p and q,
-   // species part: the species fluxes into Flux[0..NS-1] and the heat flux
-   // (conduction + species enthalpy transport) returned
-   __device__ double species_at(double *Flux, const P3 &p, const P3 &q) const
+   // The cell values the species part of a face reads, loaded once per cell:
+   // a line walk hands the right cell's values on as the next face's left.
+   struct SpCell {
+      double X[NS], Di[NS], rY[NS];
+      double rho, T, lam, W;      // W: molar weight of the mixture from X
+   };
+   __device__ void sp_load(SpCell &c, const P3 &p) const
    {
-      const int t = ld<int>(f.nType[DIR], p);
-      const double m_s = ld<double>(f.m_s[DIR], p);
-      double Xp[NS], Xq[NS];
 #pragma unroll
-      for (int i = 0; i < NS; i++) { Xp[i] = ld<double>(f.MolarFracs, p, i); Xq[i] = ld<double>(f.MolarFracs, q, i); }
-      const double rho_s = interp2(t, ld<double>(f.rho, p), ld<double>(f.rho, q));
-      const double iMixW_s = 1.0/interp2(t, MX::MolarWeightFromX(m, Xp), MX::MolarWeightFromX(m, Xq));
-      const double Tp = ld<double>(f.temperature, p), Tq = ld<double>(f.temperature, q);
-      const double T_s = interp2(t, Tp, Tq);
-      double heatFlux = interp2(t, ld<double>(f.lam, p), ld<double>(f.lam, q))*m_s*(Tp - Tq);
+      for (int i = 0; i < NS; i++) {
+         c.X[i] = ld<double>(f.MolarFracs, p, i);
+         c.Di[i] = ld<double>(f.Di, p, i);
+         c.rY[i] = ld<double>(f.Conserved, p, i);
+      }
+      c.rho = ld<double>(f.rho, p);
+      c.T = ld<double>(f.temperature, p);
+      c.lam = ld<double>(f.lam, p);
+      c.W = MX::MolarWeightFromX(m, c.X);
+   }
+
+   // GetDiffusionFlux (rhs.inl:1140-1218) at the face between cells P and Q
+   // (node type t and staggered metric m_s of the left cell), species part:
+   // the species fluxes into Flux[0..NS-1] and the heat flux (conduction +
+   // species enthalpy transport) returned
+   __device__ double species_cells(double *Flux, int t, double m_s, const SpCell &P, const SpCell &Q) const
+   {
+      const double rho_s = interp2(t, P.rho, Q.rho);
+      const double iMixW_s = 1.0/interp2(t, P.W, Q.W);
+      const double T_s = interp2(t, P.T, Q.T);
+      double heatFlux = interp2(t, P.lam, Q.lam)*m_s*(P.T - Q.T);
       double ViCorr = 0.0;
       double hs[NS];
 #pragma unroll
       for (int i = 0; i < NS; i++) {
          hs[i] = MX::SpeciesEnthalpy(m, i, T_s);
-         const double YiVi = interp2(t, ld<double>(f.Di, p, i), ld<double>(f.Di, q, i))*m_s*(Xp[i] - Xq[i])*
-                             MX::SpeciesW(m, i)*iMixW_s;
+         const double YiVi = interp2(t, P.Di[i], Q.Di[i])*m_s*(P.X[i] - Q.X[i])*MX::SpeciesW(m, i)*iMixW_s;
          Flux[i] = -rho_s*YiVi;
          heatFlux += rho_s*YiVi*hs[i];
          ViCorr += YiVi;
       }
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-         const double corr = interp2(t, ld<double>(f.Conserved, p, i), ld<double>(f.Conserved, q, i))*ViCorr;
+         const double corr = interp2(t, P.rY[i], Q.rY[i])*ViCorr;
          Flux[i] += corr;
          heatFlux -= corr*hs[i];
       }
       return heatFlux;
    }
 
-   // GetDiffusionFlux, momentum part: the viscous stress into Flux[0..2]
-   // and u.sigma returned (GetSigma, rhs.inl:1058-1137)
+   __device__ double species_at(double *Flux, const P3 &p, const P3 &q) const
+   {
+      SpCell P, Q;
+      sp_load(P, p);
+      sp_load(Q, q);
+      return species_cells(Flux, ld<int>(f.nType[DIR], p), ld<double>(f.m_s[DIR], p), P, Q);
+   }
+
+   // the cell values the momentum part of a face reads
+   struct StCell { double v[3], mu; };
+   __device__ void st_load(StCell &c, const P3 &p) const
+   {
+#pragma unroll
+      for (int i = 0; i < 3; i++) c.v[i] = vel(p, i);
+      c.mu = ld<double>(f.mu, p);
+   }
+
    __device__ double stress_at(double *Flux, const P3 &p, const P3 &q, const double g1[2][3],
                                const double g2[2][3]) const
    {
-      const int t = ld<int>(f.nType[DIR], p);
-      const double m_s = ld<double>(f.m_s[DIR], p);
-      const double mu_s = interp2(t, ld<double>(f.mu, p), ld<double>(f.mu, q));
-      double vp[3], vq[3], dn[3], s[3];
+      StCell P, Q;
+      st_load(P, p);
+      st_load(Q, q);
+      return stress_cells(Flux, ld<int>(f.nType[DIR], p), ld<double>(f.m_s[DIR], p), P, Q, g1, g2);
+   }
+
+   // GetDiffusionFlux, momentum part: the viscous stress into Flux[0..2]
+   // and u.sigma returned (GetSigma, rhs.inl:1058-1137)
+   __device__ double stress_cells(double *Flux, int t, double m_s, const StCell &P, const StCell &Q,
+                                  const double g1[2][3], const double g2[2][3]) const
+   {
+      const double mu_s = interp2(t, P.mu, Q.mu);
+      const double *vp = P.v, *vq = Q.v;
+      double dn[3], s[3];
 #pragma unroll
-      for (int i = 0; i < 3; i++) { vp[i] = vel(p, i); vq[i] = vel(q, i); dn[i] = m_s*(vq[i] - vp[i]); }
+      for (int i = 0; i < 3; i++) dn[i] = m_s*(vq[i] - vp[i]);
       if (DIR == 0) {
          const double dUdY = interp2(t, g1[0][0], g1[1][0]);
          const double dVdY = interp2(t, g1[0][1], g1[1][1]);
@@ -147,6 +187,9 @@ struct Diffusion {
 // from p0 only when offM1(nType[p0]) = 0, a staggered wall node).  Face j's
 // flux is differenced with face j-1's into Conserved_t of the cell between
 // them with m_d (rhs.inl:1264-1279).
+// The left cell's values of a face are the previous face's right cell's,
+// handed on in registers (reloaded at face 1, whose left cell p0 follows pm1
+// only when offM1(nType[p0]) = -1).
 // Two component groups, each walked by its own launch: G = 0 the species
 // fluxes, whose heat flux (conduction + species enthalpy transport) goes to
 // a face buffer H; G = 1 the viscous stress and the energy flux
@@ -205,18 +248,34 @@ diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    using D_t = Diffusion<MX, DIR>;
    D_t D{f, *mix, g};
    double g1[2][3], g2[2][3], Fm[K::NC], Fc[K::NC];
+   // species group: the left cell's values, handed on from the previous
+   // face's right cell (reloaded at face 1, whose left cell p0 follows pm1
+   // only when offM1(nType[p0]) = -1)
+   using Cell = std::conditional_t<G == 0, typename D_t::SpCell, typename D_t::StCell>;
+   auto load = [&](Cell &c, const P3 &p) {
+      if constexpr (G == 0) D.sp_load(c, p); else D.st_load(c, p);
+   };
+   Cell P;
    auto face = [&](const P3 &p, int jj, double *F) {
       const P3 q = wplus(g, DIR, p, 1);
-      if constexpr (G == 1) {
+      Cell Q;
+      load(Q, q);
+      const int t = ld<int>(f.nType[DIR], p);
+      const double m_s = ld<double>(f.m_s[DIR], p);
+      const uint32_t fid = Faces<DIR>::id(g, line, jj);
+      if constexpr (G == 0) {
+         const double heat = D.species_cells(F, t, m_s, P, Q);
+         if (jj >= j0) H[fid] = heat;
+      } else {
          D.template deriv3<D_t::D1>(q, g1[1]);
          D.template deriv3<D_t::D2>(q, g2[1]);
-      }
-      diff_face<MX, DIR, G>(D, F, p, q, g1, g2, H, Faces<DIR>::id(g, line, jj), jj >= j0);
-      if constexpr (G == 1) {
+         F[3] = (D.stress_cells(F, t, m_s, P, Q, g1, g2) - H[fid]);
 #pragma unroll
          for (int i = 0; i < 3; i++) { g1[0][i] = g1[1][i]; g2[0][i] = g2[1][i]; }
       }
+      P = Q;
    };
+   load(P, cell);
    if constexpr (G == 1) {
       D.template deriv3<D_t::D1>(gl, g1[0]);
       D.template deriv3<D_t::D2>(gl, g2[0]);
@@ -224,6 +283,7 @@ diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    face(cell, j, Fm);
    for (j++; j < j1; j++) {
       cell = (j == 1) ? p0 : wplus(g, DIR, cell, 1);
+      if (j == 1) load(P, cell);
       face(cell, j, Fc);
       const double m = ld<double>(f.m_d[DIR], cell);
 #pragma unroll
