@@ -243,10 +243,12 @@ def units_of(fam, name, nfaces_dir, cells, counts):
 
 def chemistry(n, dt, seed, cpu, threads, tune=True):
     """Batched implicit chemistry, 30 species / 180 reversible reactions
-    (34 x 34 Jacobians): solves/s of the FP64 and the FP32-LU + FP64-refinement
-    paths on n^3 cells, launch shapes chosen by the autotuner, FP32+IR against
-    FP64 per species, Rosenbrock steps per cell; CPU baseline: the oracle on
-    a 4^3 sample."""
+    (34 x 34 Jacobians): solves/s of four paths on n^3 cells -- FP64 with the
+    reference's central-difference Jacobian (the plain FP64 path), FP32 LU +
+    FP64 refinement with that Jacobian, and both precisions with the analytic
+    Jacobian -- launch shapes chosen by the autotuner; accuracy of every path
+    against the converged solution (the oracle at step tolerance 1e-13) on a
+    4^3 sub-sample, per species; CPU baseline: the oracle on that sample."""
     import numpy as np
     import torch
     import htr_amd as H
@@ -263,8 +265,11 @@ def chemistry(n, dt, seed, cpu, threads, tune=True):
     sub = fs.grid((0, 0, 0), (n - 1, n - 1, max(n // 4, 1) - 1))
     out = {"workload": "chem30_%d^3" % n, "cells": n ** 3, "dt": dt, "jacobian": "34x34 (nEq)",
            "mechanism": "CH4_30SpMix species, 180 synthetic reversible reactions (seed 0)"}
+    AJ = H.HTR_CHEM_JAC_ANALYTIC
+    paths = ((H.HTR_CHEM_FP64, "fp64"), (H.HTR_CHEM_FP32_IR, "fp32_ir"),
+             (H.HTR_CHEM_FP64 | AJ, "fp64_aj"), (H.HTR_CHEM_FP32_IR | AJ, "fp32_ir_aj"))
     res = {}
-    for prec, name in ((H.HTR_CHEM_FP64, "fp64"), (H.HTR_CHEM_FP32_IR, "fp32_ir")):
+    for prec, name in paths:
         if tune:
             best, table = H.chem_autotune(fs, mix, dt, prec, grid=sub,
                                           configs=[(16, 0), (32, 0), (64, 0), (64, 4)])
@@ -292,27 +297,49 @@ def chemistry(n, dt, seed, cpu, threads, tune=True):
                      "steps_p99": float(np.percentile(steps, 99)), "steps_max": int(steps.max())}
         for k in init:
             fs[k].copy_(init[k])
-    d = np.abs(res["fp32_ir"] - res["fp64"])
-    out["fp32_ir_vs_fp64"] = {"max_abs": float(d.max()),
-                              "max_rel_per_species": [float(x) for x in d.reshape(30, -1).max(axis=1) /
-                                                      np.abs(res["fp64"]).reshape(30, -1).max(axis=1)]}
-    out["speedup_fp32_ir"] = out["fp32_ir"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    H.chem_launch_config(0, 0)
+    for name in ("fp32_ir", "fp32_ir_aj", "fp64_aj"):
+        d = np.abs(res[name] - res["fp64"])
+        out[name + "_vs_fp64"] = {"max_abs": float(d.max()),
+                                  "max_rel_per_species": [float(x) for x in d.reshape(30, -1).max(axis=1) /
+                                                          np.abs(res["fp64"]).reshape(30, -1).max(axis=1)]}
+    out["speedup_fp32_ir"] = out["fp32_ir_aj"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["speedup_fp32_ir_same_jacobian"] = {"fd": out["fp32_ir"]["solves_per_s"] / out["fp64"]["solves_per_s"],
+                                            "analytic": out["fp32_ir_aj"]["solves_per_s"] / out["fp64_aj"]["solves_per_s"]}
     if cpu:
+        # the converged solution of a 4^3 sub-sample (oracle, analytic
+        # Jacobian, tolerance 1e-13) and the CPU baseline (oracle, the
+        # reference algorithm) on the same cells
         from oracle import oracle as O
         m = O.Mix.from_mechanism(meta["mech"])
         ns = 4
         g = O.Grid(ns, ns, ns)
-        f = O.Fields(g, 30)
         sl = (slice(0, ns),) * 3
-        for k in ("Conserved", "temperature"):
-            a = init[k].cpu().numpy()
-            f[k] = a[(slice(None),) + sl] if a.ndim == 4 else a[sl]
-        f["Conserved_t"] = 0.0
+
+        def sample():
+            f = O.Fields(g, 30)
+            for k in ("Conserved", "temperature"):
+                a = init[k].cpu().numpy()
+                f[k] = a[(slice(None),) + sl] if a.ndim == 4 else a[sl]
+            f["Conserved_t"] = 0.0
+            return f
+        f = sample()
         t0 = time.perf_counter()
         O.chem_update(g, f, dt, m)
         el = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": ns ** 3 / el, "unit": "solves/s", "cores": threads, "kind": "port",
                                "sample": "4^3 cells of the same workload, %.2f s" % el}
+        fc = sample()
+        O.chem_update(g, fc, dt, m, jacobian=O.JAC_ANALYTIC, tol=1e-13)
+        conv = fc["Conserved"][:30]
+        scale = np.abs(conv).reshape(30, -1).max(axis=1)
+        acc = {}
+        for name in res:
+            d = np.abs(res[name][(slice(None),) + sl] - conv).reshape(30, -1).max(axis=1)
+            acc[name] = {"max_abs": float(d.max()), "max_rel_per_species": [float(x) for x in d / scale]}
+        out["error_vs_converged_4^3"] = acc
+        print("chemistry error vs converged: %s" % {k: "%.2e" % v["max_abs"] for k, v in acc.items()},
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -448,7 +475,7 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out = chemistry(max(a.chem_n, 4), a.chem_dt, a.seed, not a.no_cpu, threads, a.chem_tune)
         out.update({"metric": "implicit-chemistry solves/s (30 species, 34x34 Jacobians), FP32+IR vs FP64",
-                    "value": out["fp32_ir"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+                    "value": out["fp32_ir_aj"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
                     "higher_is_better": True, "dtype": "f64 (FP32 LU + FP64 refinement)", "data": "synthetic"})
         print(json.dumps(out), flush=True)
         return

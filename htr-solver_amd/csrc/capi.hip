@@ -800,7 +800,10 @@ int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int preci
    if (diag) ns = diag->nsteps;
    const int esz4 = 4;
    Geo geo; htr_fields fr; int rc = prep(g, f, m, fr, geo, &ns, 1, &esz4); if (rc) return done(rc);
-   if (precision != HTR_CHEM_FP64 && precision != HTR_CHEM_FP32_IR) return set_error(HTR_E_ARG, "bad precision %d", precision);
+   {
+      const int p0 = precision & ~HTR_CHEM_JAC_ANALYTIC;
+      if (p0 != HTR_CHEM_FP64 && p0 != HTR_CHEM_FP32_IR) return set_error(HTR_E_ARG, "bad precision %d", precision);
+   }
    HTR_NEED(Conserved); HTR_NEED(Conserved_t); HTR_NEED(Conserved_t_old); HTR_NEED(temperature);
    switch (kind_of(m)) {
       case K_AIR: return chem_update_launch<MixAir>(fr, m->dev, geo, dt, precision, ns, m->host.nReac, (hipStream_t)stream);

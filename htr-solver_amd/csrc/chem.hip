@@ -34,7 +34,21 @@
 //                     of the solution; a solve whose refinement stalls makes
 //                     the step fall back to an FP64 LU of the same matrix,
 //                     factorised in place of the Jacobian (which is then
-//                     formed again if the step is retried).
+//                     formed again if the step is retried).  With 64 lanes per
+//                     cell the refinement runs in registers (one row per lane,
+//                     operands broadcast by readlane); the rounding order of
+//                     its triangular solves is free, the refinement fixes it.
+//
+// Two Jacobians (HTR_CHEM_JAC_ANALYTIC or-ed into the precision):
+//   central differences  the reference's, 2 nEq right-hand sides;
+//   analytic             the exact derivative of the same right-hand side
+//                        (no reference counterpart): per reaction the
+//                        derivatives of its rate of progress with respect to
+//                        T, ctot = P/(R T) and its species' concentrations;
+//                        per species row the sum over its reactions; the
+//                        temperature enters through dT/dx from the energy
+//                        relation.  One right-hand side and one Jacobian pass
+//                        per step instead of 2 nEq + 1 right-hand sides.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -66,7 +80,9 @@ struct ChemLayout {
    static constexpr int SR = MISC + 4;                 // reaction rates, nReac entries
    // the factors follow the nReac reaction rates: A = SR + nReac
    static constexpr int a_doubles() { return (PREC == HTR_CHEM_FP64) ? N*N : (N*N + 1)/2; }
-   __host__ __device__ static int cell_doubles(int nreac) { return SR + nreac + a_doubles(); }
+   // the analytic Jacobian's per-reaction derivatives (4 nReac) follow the factors
+   __host__ __device__ static int cell_doubles(int nreac, bool aj)
+   { return SR + nreac + a_doubles() + (aj ? 4*nreac : 0); }
 };
 
 template<class MX, int PREC, int L>
@@ -83,13 +99,15 @@ struct ChemGroup {
    bool    jac_valid;  // Jacobian present in LDS (FP32_IR: a fallback overwrites it)
    bool    lu64;       // FP32_IR: this step's matrix is factorised in FP64 (in the Jacobian's place)
    int     fallbacks;
+   bool    aj;         // analytic Jacobian
+   double *rq;         // its per-reaction derivatives (4 nReac doubles)
 
-   __device__ double *v(int k) const { return s + Lay::VEC + k*N; }
-   __device__ double *Jm() const { return s + Lay::J; }
-   __device__ int *ind() const { return reinterpret_cast<int*>(s + Lay::IND); }
+   __device__ __forceinline__ double *v(int k) const { return s + Lay::VEC + k*N; }
+   __device__ __forceinline__ double *Jm() const { return s + Lay::J; }
+   __device__ __forceinline__ int *ind() const { return reinterpret_cast<int*>(s + Lay::IND); }
 
    // in-order sum of n terms in LDS (every lane forms the same sum)
-   __device__ static double sum_in_order(const double *t, int n)
+   __device__ __forceinline__ static double sum_in_order(const double *t, int n)
    {
       double a = 0.0;
 #pragma unroll 4
@@ -98,7 +116,7 @@ struct ChemGroup {
    }
 
    // group maximum (exact in any order)
-   __device__ static double gmax(double x)
+   __device__ __forceinline__ static double gmax(double x)
    {
 #pragma unroll
       for (int o = L/2; o > 0; o >>= 1) x = dmax(x, __shfl_xor(x, o, L));
@@ -108,7 +126,7 @@ struct ChemGroup {
    // MultiComponent.inl:187-210 over the group: the species terms of the
    // energy and heat-capacity sums are formed by their lanes, the sums in
    // order by every lane
-   __device__ double t_from_e(double e0, double Tg)
+   __device__ __forceinline__ double t_from_e(double e0, double Tg)
    {
       const double *Y = s + Lay::SY;
       double *ta = s + Lay::SA, *tb = s + Lay::SB;
@@ -132,7 +150,7 @@ struct ChemGroup {
    }
 
    // ImplicitSolver::rhs (chem.hpp:53-93) of x (LDS) into r (LDS)
-   __device__ void rhs(double *r, const double *x)
+   __device__ __forceinline__ void rhs(double *r, const double *x)
    {
       double *Y = s + Lay::SY, *ta = s + Lay::SA, *tb = s + Lay::SB, *w = s + Lay::SW, *Rr = s + Lay::SR;
       const double *ct = v(Lay::CT);
@@ -210,7 +228,7 @@ struct ChemGroup {
    }
 
    // Rosenbrock::GetJacobian (math_utils.hpp:249-269), column by column
-   __device__ void jacobian(double *x)
+   __device__ __forceinline__ void jacobian(double *x)
    {
       double *fp = v(Lay::FP), *fm = v(Lay::FM), *J = Jm();
       for (int j = 0; j < N; j++) {
@@ -234,8 +252,150 @@ struct ChemGroup {
       jac_valid = true;
    }
 
+   // Analytic Jacobian of rhs() at x (no reference counterpart; the
+   // reference's is jacobian() above).  Needs the temperature of x in T and
+   // its clamped mass fractions in LDS: called right after rhs(.., x).
+   //   C_k = rho Y_k rhoRef/W_k (a clamped Y_k moves only with rho);
+   //   ctot = P/(R T) = sum C_k, and log T + log(R/P) = -log ctot;
+   //   dT/dx_j = (de/dx_j - sum_k e_k dY_k/dx_j)/cv;
+   //   dKf/dT = Kf (n/T + (E/R)/T^2), d(G_k/RT)/dT = -(H_k/RT)/T;
+   //   J(i, j) = W_i iwiRef sum_r nu_ir dq_r/dx_j, rows nSpec.. zero.
+   __device__ __forceinline__ void jacobian_analytic(const double *x)
+   {
+      double *J = Jm();
+      double *C = s + Lay::SA, *Gk = s + Lay::SB, *Yc = s + Lay::SY;
+      double *ek = v(Lay::RR), *dG = v(Lay::RB), *tcv = v(Lay::VV), *tse = v(Lay::RS), *tcw = v(Lay::CS);
+      double *fr = v(Lay::G1), *dT = v(Lay::FP), *dct = v(Lay::FM);
+      const int nR = m.nReac;
+      double *QT = rq, *QC = rq + nR, *CKF = rq + 2*nR, *CKB = rq + 3*nR, *Dq = s + Lay::SR;
+      const double rho = sum_in_order(x, NS);
+      const double irho = 1.0/rho;
+      double vel[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) vel[i] = x[NS+i]*irho;
+      const double k2 = vel[0]*vel[0] + vel[1]*vel[1] + vel[2]*vel[2];
+      const double e = x[NS+3]*irho - 0.5*k2;
+      const double Tr = T*m.TRef, rr = rho*m.rhoRef;
+      for (int i = lane; i < NS; i += L) {
+         const DSpec &sp = m.sp[i];
+         const double y = irho*x[i];
+         const double yc = dmin(dmax(y, 1e-60), 1.0);
+         const bool f = (yc == y);
+         Yc[i] = yc;
+         fr[i] = f ? 1.0 : 0.0;
+         const double h = sp_enthalpy(sp, Tr);
+         ek[i] = (h - RGAS*Tr/sp.W)*m.ieRef;
+         tcv[i] = yc*(sp_cp(sp, Tr) - RGAS/sp.W)*m.iCpRef;
+         tse[i] = f ? ek[i]*yc : 0.0;
+         tcw[i] = f ? 0.0 : yc/sp.W;
+         C[i] = yc*rr/sp.W;
+         Gk[i] = sp_free_enthalpy(sp, Tr);
+         dG[i] = -(h*sp.W/RGAS)/(Tr*Tr);
+      }
+      gsync();
+      const double icv = irho/sum_in_order(tcv, NS);
+      const double sfe = sum_in_order(tse, NS), clampW = sum_in_order(tcw, NS), ctot = sum_in_order(C, NS);
+      const double lct = log(ctot);
+      for (int j = lane; j < N; j += L) {
+         double a, b = 0.0;
+         if (j < NS) {
+            const bool f = (fr[j] != 0.0);
+            a = ((0.5*k2 - e) - ((f ? ek[j] : 0.0) - sfe))*icv;
+            b = m.rhoRef*((f ? 1.0/m.sp[j].W : 0.0) + clampW);
+         } else if (j < NS + 3) a = -((j == NS) ? vel[0] : (j == NS + 1) ? vel[1] : vel[2])*icv;
+         else                   a = icv;
+         dT[j] = a;
+         dct[j] = b;
+      }
+      // per reaction: dq/dT (per nondimensional T), dq/dctot, c Kf, c Kb and
+      // Kf a - Kb b (third-body term)
+      for (int k = lane; k < nR; k += L) {
+         const DReac &rc = m.rc[k];
+         const double Kf = MX::RateCoeff(rc.A, rc.n, rc.EovR, Tr);
+         double dlKf = 0.0;
+         if (rc.n != 0.0f) dlKf += (double)rc.n/Tr;
+         if (fabs((double)rc.EovR) > 1e-5) dlKf += (double)rc.EovR/(Tr*Tr);
+         double a = 1.0;
+         for (int i = 0; i < rc.Ne; i++) {
+            const double c0 = C[rc.ed_ind[i]];
+            a *= (rc.ed_nu[i] == 1.0f) ? c0 : pow(c0, (double)rc.ed_nu[i]);
+         }
+         double Kb = 0.0, b = 1.0, sumNu = 0.0, dlKb = 0.0;
+         if (rc.has_backward) {
+            double sumNuG = 0.0, sdG = 0.0;
+            for (int i = 0; i < rc.Ne; i++) {
+               const int id = rc.ed_ind[i];
+               sumNu -= rc.ed_nu[i]; sumNuG -= rc.ed_nu[i]*Gk[id]; sdG -= rc.ed_nu[i]*dG[id];
+            }
+            for (int i = 0; i < rc.Np; i++) {
+               const int id = rc.pd_ind[i];
+               sumNu += rc.pd_nu[i]; sumNuG += rc.pd_nu[i]*Gk[id]; sdG += rc.pd_nu[i]*dG[id];
+               b *= (rc.pd_nu[i] == 1.0f) ? C[id] : pow(C[id], (double)rc.pd_nu[i]);
+            }
+            Kb = Kf*exp(sumNuG - sumNu*lct);
+            dlKb = dlKf + sdG;
+         }
+         const double D = Kf*a - Kb*b;
+         const bool tb = (rc.kind == REAC_THIRDB && rc.Ntb > 0);
+         double c = 1.0;
+         if (tb) {
+            c = ctot;
+            for (int i = 0; i < rc.Ntb; i++) c += C[rc.tb_ind[i]]*(rc.tb_eff[i] - 1.0);
+         }
+         QT[k] = c*(Kf*dlKf*a - Kb*dlKb*b)*m.TRef;
+         QC[k] = c*Kb*b*sumNu/ctot + (tb ? D : 0.0);
+         CKF[k] = c*Kf;
+         CKB[k] = c*Kb;
+         Dq[k] = tb ? D : 0.0;
+      }
+      gsync();
+      // rows: lane over species rows; the sparse part of row i (the
+      // concentrations of the species of its reactions) is accumulated into
+      // the row, the dense parts (dT, dctot, clamped species) added last
+      for (int i = lane; i < N; i += L) {
+         for (int j = 0; j < N; j++) J[i + j*N] = 0.0;
+         if (i >= NS) continue;
+         double aT = 0.0, aC = 0.0, scl = 0.0;
+         for (int e2 = m.spr_ptr[i]; e2 < m.spr_ptr[i+1]; e2++) {
+            const int r = m.spr_reac[e2];
+            const double nu = m.spr_nu[e2];
+            const DReac &rc = m.rc[r];
+            aT += nu*QT[r];
+            aC += nu*QC[r];
+            for (int q = 0; q < rc.Ne; q++) {
+               double p = (rc.ed_nu[q] == 1.0f) ? 1.0 : (double)rc.ed_nu[q]*pow(C[rc.ed_ind[q]], (double)rc.ed_nu[q] - 1.0);
+               for (int l = 0; l < rc.Ne; l++)
+                  if (l != q) p *= (rc.ed_nu[l] == 1.0f) ? C[rc.ed_ind[l]] : pow(C[rc.ed_ind[l]], (double)rc.ed_nu[l]);
+               const int sp = rc.ed_ind[q];
+               const double val = nu*CKF[r]*p*m.rhoRef/m.sp[sp].W;
+               if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
+            }
+            if (rc.has_backward)
+               for (int q = 0; q < rc.Np; q++) {
+                  double p = (rc.pd_nu[q] == 1.0f) ? 1.0 : (double)rc.pd_nu[q]*pow(C[rc.pd_ind[q]], (double)rc.pd_nu[q] - 1.0);
+                  for (int l = 0; l < rc.Np; l++)
+                     if (l != q) p *= (rc.pd_nu[l] == 1.0f) ? C[rc.pd_ind[l]] : pow(C[rc.pd_ind[l]], (double)rc.pd_nu[l]);
+                  const int sp = rc.pd_ind[q];
+                  const double val = -nu*CKB[r]*p*m.rhoRef/m.sp[sp].W;
+                  if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
+               }
+            if (rc.kind == REAC_THIRDB && rc.Ntb > 0)
+               for (int q = 0; q < rc.Ntb; q++) {
+                  const int sp = rc.tb_ind[q];
+                  const double val = nu*Dq[r]*(rc.tb_eff[q] - 1.0)*m.rhoRef/m.sp[sp].W;
+                  if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
+               }
+         }
+         const double sc = m.sp[i].W*m.iwiRef;
+         for (int j = 0; j < N; j++)
+            J[i + j*N] = sc*(J[i + j*N] + aT*dT[j] + aC*dct[j] + ((j < NS) ? scl : 0.0));
+      }
+      gsync();
+      jac_valid = true;
+   }
+
    // power of two nearest to 1/a (exact scaling)
-   __device__ static double pow2_inv(double a)
+   __device__ __forceinline__ static double pow2_inv(double a)
    {
       if (!(a > 0.0) || !(a < 1.0e308)) return 1.0;
       // a = m 2^e with m in [0.5, 1): e from the biased exponent (normal a)
@@ -246,7 +406,7 @@ struct ChemGroup {
 
    // LUdec::ludcmp (math_utils.hpp:87-155) of the N x N column-major matrix M
    template<typename T_>
-   __device__ void ludcmp(T_ *M)
+   __device__ __forceinline__ void ludcmp(T_ *M)
    {
       double *vv = v(Lay::VV);
       int *piv = ind();
@@ -305,7 +465,7 @@ struct ChemGroup {
    // reference's order), back substitution row by row by one lane in the
    // reference's order
    template<typename T_, typename V>
-   __device__ void lubksb(const T_ *M, V *b)
+   __device__ __forceinline__ void lubksb(const T_ *M, V *b)
    {
       const int *piv = ind();
       if (lane == 0)
@@ -326,10 +486,13 @@ struct ChemGroup {
    }
 
    // A = I/(gam dt) - J, factorised (Rosenbrock::step, math_utils.hpp:282-291)
-   __device__ void factor(double dt, double *x)
+   __device__ __forceinline__ void factor(double dt, double *x)
    {
       const double d = 1.0/((1.0/2.0)*dt);
-      if (!jac_valid) jacobian(x);     // after an FP64 fallback: formed again at the step's start point
+      if (!jac_valid) {                // after an FP64 fallback: formed again at the step's start point
+         if (aj) { rhs(v(Lay::G4), x); jacobian_analytic(x); }
+         else jacobian(x);
+      }
       if (PREC == HTR_CHEM_FP64) {
          double *M = reinterpret_cast<double*>(A);
          const double *J = Jm();
@@ -367,7 +530,7 @@ struct ChemGroup {
    }
 
    // FP32_IR fallback: the FP64 LU of the same matrix, in the Jacobian's place
-   __device__ void factor64_in_place(double dt)
+   __device__ __forceinline__ void factor64_in_place(double dt)
    {
       const double d = 1.0/((1.0/2.0)*dt);
       double *M = Jm();
@@ -381,11 +544,82 @@ struct ChemGroup {
       fallbacks++;
    }
 
+   // value of v in lane l of the wave (l uniform)
+   __device__ __forceinline__ static double rdlane(double x, int l)
+   {
+      const long long b = __double_as_longlong(x);
+      const int lo = __builtin_amdgcn_readlane((int)b, l);
+      const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+      return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+   }
+
+   // FP32_IR refinement with one row per lane (L == 64 >= N): the solution,
+   // the residual and the correction in registers, the FP32 factors and the
+   // FP64 Jacobian read from LDS, operands broadcast by readlane.  Returns
+   // false when the refinement stalls (b then holds the right-hand side).
+   __device__ __forceinline__ bool solve_ir_reg(double *b, double dt)
+   {
+      const float *M = reinterpret_cast<const float*>(A);
+      const double *J = Jm();
+      const double *rs = v(Lay::RS), *cs = v(Lay::CS);
+      const int *piv = ind();
+      const int i = lane;
+      const bool act = i < N;
+      const double d = 1.0/((1.0/2.0)*dt);
+      const double rb = act ? b[i] : 0.0;
+      const double rsi = act ? rs[i] : 0.0, csi = act ? cs[i] : 0.0;
+      double xk = 0.0, prev = 0.0;
+      bool done = false;
+      for (int it = 0; it < 9 && !done; it++) {
+         double r = rb;
+         if (it > 0) {
+            for (int j = 0; j < N; j++) {
+               const double xj = rdlane(xk, j);
+               if (act) r -= (((i == j) ? d : 0.0) - J[i + j*N])*xj;
+            }
+         }
+         r *= rsi;
+         for (int k = 0; k < N; k++) {          // row interchanges, in order
+            const int ip = __builtin_amdgcn_readfirstlane(piv[k]);
+            const double vk = rdlane(r, k), vp = rdlane(r, ip);
+            if (i == k) r = vp;
+            if (i == ip) r = vk;
+         }
+         for (int k = 0; k < N - 1; k++) {      // unit lower triangle
+            const double bk = rdlane(r, k);
+            if (act && i > k) r -= (double)M[i + k*N]*bk;
+         }
+         for (int k = N - 1; k >= 0; k--) {     // upper triangle
+            const double xkk = rdlane(r, k)/(double)M[k + k*N];
+            if (i == k) r = xkk;
+            else if (act && i < k) r -= (double)M[i + k*N]*xkk;
+         }
+         const double ci = act ? csi*r : 0.0;
+         xk += ci;
+         const double nc = gmax(fabs(ci)), nx = gmax(fabs(xk));
+         if (it == 0) { prev = nc; continue; }
+         if (!(nc > 1e-15*nx)) done = true;
+         else if (!(nc < 0.5*prev)) break;
+         prev = nc;
+      }
+      if (done && act) b[i] = xk;
+      gsync();
+      return done;
+   }
+
    // b <- A^-1 b
-   __device__ void solve(double *b, double dt)
+   __device__ __forceinline__ void solve(double *b, double dt)
    {
       if (PREC == HTR_CHEM_FP64) { lubksb<double, double>(reinterpret_cast<double*>(A), b); return; }
       if (lu64) { lubksb<double, double>(Jm(), b); return; }
+      if (L == 64 && N <= 64) {
+         if (!solve_ir_reg(b, dt)) {
+            // refinement stalled: FP64 LU of the same matrix for this step
+            factor64_in_place(dt);
+            lubksb<double, double>(Jm(), b);
+         }
+         return;
+      }
       const float *M = reinterpret_cast<const float*>(A);
       const double *J = Jm();
       const double *rs = v(Lay::RS), *cs = v(Lay::CS);
@@ -431,7 +665,7 @@ struct ChemGroup {
    }
 
    // Rosenbrock::step (math_utils.hpp:282-330): Kaps-Rentrop, gamma = 1/2
-   __device__ double step(double *x, double dt, double t0)
+   __device__ __forceinline__ double step(double *x, double dt, double t0)
    {
       double *dx = v(Lay::DX), *xsav = v(Lay::XSAV), *dxsav = v(Lay::DXSAV), *err = v(Lay::ERR);
       double *g1 = v(Lay::G1), *g2 = v(Lay::G2), *g3 = v(Lay::G3), *g4 = v(Lay::G4);
@@ -465,7 +699,7 @@ struct ChemGroup {
    }
 
    // Rosenbrock::itercheck (math_utils.hpp:337-360)
-   __device__ bool itercheck(double &dt, double &dtNext, bool finish)
+   __device__ __forceinline__ bool itercheck(double &dt, double &dtNext, bool finish)
    {
       const double *err = v(Lay::ERR);
       double errmax = 0.0;
@@ -485,7 +719,7 @@ struct ChemGroup {
 
    // Rosenbrock::solve (math_utils.hpp:195-247, 362-386); returns accepted
    // steps, or -1 if a reference cap was hit
-   __device__ int solve_cell(double dtTry, double DelT)
+   __device__ __forceinline__ int solve_cell(double dtTry, double DelT)
    {
       double *x = v(Lay::X), *dx = v(Lay::DX), *xsav = v(Lay::XSAV), *dxsav = v(Lay::DXSAV);
       bool finish = false;
@@ -494,8 +728,13 @@ struct ChemGroup {
       for (int st = 0; st < 100000; st++) {
          const double t0 = time;
          double dtNext = 0.0;
-         jacobian(x);
-         rhs(dx, x);
+         if (aj) {                     // the right-hand side first: the Jacobian needs T(x)
+            rhs(dx, x);
+            jacobian_analytic(x);
+         } else {
+            jacobian(x);
+            rhs(dx, x);
+         }
          for (int i = lane; i < N; i += L) { xsav[i] = x[i]; dxsav[i] = dx[i]; }
          gsync();
          int jtry;
@@ -519,7 +758,7 @@ template<class MX, int PREC, int L>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 chem_group_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double dt,
                   htr_view nsteps, int *__restrict__ fails, int *__restrict__ queue, int cell_doubles,
-                  int64_t n)
+                  int64_t n, bool aj)
 {
    HIP_DYNAMIC_SHARED(double, lds)
    using Lay = ChemLayout<MX, PREC>;
@@ -527,8 +766,9 @@ chem_group_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    const int grp = threadIdx.x/L;
    const int lane = threadIdx.x % L;
    double *s = lds + grp*cell_doubles;
-   ChemGroup<MX, PREC, L> G{*mix, s, reinterpret_cast<typename ChemGroup<MX, PREC, L>::R*>(s + Lay::SR + mix->nReac),
-                            lane, 0.0, false, false, 0};
+   const int nR = mix->nReac;
+   ChemGroup<MX, PREC, L> G{*mix, s, reinterpret_cast<typename ChemGroup<MX, PREC, L>::R*>(s + Lay::SR + nR),
+                            lane, 0.0, false, false, 0, aj, s + Lay::SR + nR + Lay::a_doubles()};
    int *slot = reinterpret_cast<int*>(s + Lay::MISC);
    for (;;) {
       if (lane == 0) slot[0] = atomicAdd(queue, 1);
@@ -577,32 +817,34 @@ int chem_set_launch(int lanes, int blocks_per_cu)
 
 template<class MX, int PREC, int L>
 static int chem_launch_l(const htr_fields &f, const DMix *mix, const Geo &g, double dt, const htr_view &nsteps,
-                         Workspace &w, int nreac, int64_t n, hipStream_t s)
+                         Workspace &w, int nreac, int64_t n, bool aj, hipStream_t s)
 {
    using Lay = ChemLayout<MX, PREC>;
-   const int cd = Lay::cell_doubles(nreac);
+   const int cd = Lay::cell_doubles(nreac, aj);
    const size_t shm = (size_t)(64/L)*cd*sizeof(double);
    if (shm > 160*1024) return set_error(HTR_E_UNSUPPORTED, "chem: %zu B of LDS per wave", shm);
    int per_cu = g_chem_blocks_per_cu;
    if (per_cu == 0) per_cu = std::max(1, std::min(16, (int)((160*1024)/shm)));
    const int64_t groups = (int64_t)256*per_cu*(64/L);
    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 64/L - 1)/(64/L), groups/(64/L)));
-   HTR_LAUNCH(PREC == HTR_CHEM_FP64 ? "chem_update_fp64" : "chem_update_fp32ir", (chem_group_kernel<MX, PREC, L>),
-              dim3(blocks), dim3(64), shm, s, f, mix, g, dt, nsteps, w.counter, w.counter + 3, cd, n);
+   HTR_LAUNCH(PREC == HTR_CHEM_FP64 ? (aj ? "chem_update_fp64_aj" : "chem_update_fp64")
+                                    : (aj ? "chem_update_fp32ir_aj" : "chem_update_fp32ir"),
+              (chem_group_kernel<MX, PREC, L>),
+              dim3(blocks), dim3(64), shm, s, f, mix, g, dt, nsteps, w.counter, w.counter + 3, cd, n, aj);
    return check_launch("chem_group_kernel");
 }
 
 template<class MX, int PREC>
 static int chem_launch_p(const htr_fields &f, const DMix *mix, const Geo &g, double dt, const htr_view &nsteps,
-                         Workspace &w, int nreac, int64_t n, hipStream_t s)
+                         Workspace &w, int nreac, int64_t n, bool aj, hipStream_t s)
 {
    int lanes = g_chem_lanes;
    if (lanes == 0) lanes = (MX::NS >= 16) ? 64 : 16;
    switch (lanes) {
-      case 8:  return chem_launch_l<MX, PREC, 8>(f, mix, g, dt, nsteps, w, nreac, n, s);
-      case 16: return chem_launch_l<MX, PREC, 16>(f, mix, g, dt, nsteps, w, nreac, n, s);
-      case 32: return chem_launch_l<MX, PREC, 32>(f, mix, g, dt, nsteps, w, nreac, n, s);
-      default: return chem_launch_l<MX, PREC, 64>(f, mix, g, dt, nsteps, w, nreac, n, s);
+      case 8:  return chem_launch_l<MX, PREC, 8>(f, mix, g, dt, nsteps, w, nreac, n, aj, s);
+      case 16: return chem_launch_l<MX, PREC, 16>(f, mix, g, dt, nsteps, w, nreac, n, aj, s);
+      case 32: return chem_launch_l<MX, PREC, 32>(f, mix, g, dt, nsteps, w, nreac, n, aj, s);
+      default: return chem_launch_l<MX, PREC, 64>(f, mix, g, dt, nsteps, w, nreac, n, aj, s);
    }
 }
 
@@ -616,8 +858,10 @@ int chem_update_launch(const htr_fields &f, const DMix *mix, const Geo &g, doubl
    if (!wp) return HTR_E_DEVICE;
    Workspace &w = *wp;
    if (int rc0 = zero_counters(w.counter, s)) return rc0;      // failures and the cell queue
-   int rc = (precision == HTR_CHEM_FP64) ? chem_launch_p<MX, HTR_CHEM_FP64>(f, mix, g, dt, nsteps, w, nreac, n, s)
-                                         : chem_launch_p<MX, HTR_CHEM_FP32_IR>(f, mix, g, dt, nsteps, w, nreac, n, s);
+   const bool aj = (precision & HTR_CHEM_JAC_ANALYTIC) != 0;
+   const int prec = precision & ~HTR_CHEM_JAC_ANALYTIC;
+   int rc = (prec == HTR_CHEM_FP64) ? chem_launch_p<MX, HTR_CHEM_FP64>(f, mix, g, dt, nsteps, w, nreac, n, aj, s)
+                                    : chem_launch_p<MX, HTR_CHEM_FP32_IR>(f, mix, g, dt, nsteps, w, nreac, n, aj, s);
    if (rc) return rc;
    int fails = 0;
    if (hipMemcpyAsync(&fails, w.counter, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("HTR_AMD_LIBRARY") or os.path.join(PKG, "libhtr_amd.so
 
 HTR_EULER_HYBRID, HTR_EULER_TENO, HTR_EULER_TENOA, HTR_EULER_TENOLAD, HTR_EULER_KG = range(5)
 HTR_CHEM_FP64, HTR_CHEM_FP32_IR = 0, 1
+HTR_CHEM_JAC_ANALYTIC = 0x10   # or-ed into the precision: analytic Rosenbrock Jacobian
 HTR_BC_DIRICHLET, HTR_BC_ADIABATIC_WALL, HTR_BC_ISOTHERMAL_WALL, HTR_BC_NSCBC_INFLOW, HTR_BC_NSCBC_OUTFLOW = 0, 1, 2, 3, 4
 ERRORS = {-1: "HTR_E_ARG", -2: "HTR_E_UNSUPPORTED", -3: "HTR_E_DEVICE", -4: "HTR_E_FIELD", -5: "HTR_E_NUMERIC"}
 

@@ -63,6 +63,10 @@ extern "C" {
 /* implicit-chemistry linear-solve precision */
 #define HTR_CHEM_FP64      0   /* FP64 LU, as the reference */
 #define HTR_CHEM_FP32_IR   1   /* FP32 LU + FP64 residual iterative refinement */
+/* or-ed into the precision: the Rosenbrock Jacobian as the exact derivative of
+ * the chemistry right-hand side instead of the reference's central
+ * differences (src/Utils/math_utils.hpp:249-270); no reference counterpart */
+#define HTR_CHEM_JAC_ANALYTIC 0x10
 
 typedef struct {
    void   *ptr;                /* address of element (0,0,0), component 0 */
@@ -251,8 +255,9 @@ int htr_diffusion_flux(int dir, const htr_grid *g, const htr_fields *f, const ht
 /* AddChemistrySourcesTask::gpu_base_impl (src/prometeo_chem.cu:131-162) */
 int htr_chem_source(const htr_grid *g, const htr_fields *f, const htr_mix *m, void *stream);
 /* UpdateChemistryTask::gpu_base_impl (src/prometeo_chem.cu:65-98);
- * precision HTR_CHEM_FP64 or HTR_CHEM_FP32_IR.  Returns HTR_E_NUMERIC (after
- * finishing every cell) when any cell hit the reference's step caps. */
+ * precision HTR_CHEM_FP64 or HTR_CHEM_FP32_IR, optionally or-ed with
+ * HTR_CHEM_JAC_ANALYTIC.  Returns HTR_E_NUMERIC (after finishing every cell)
+ * when any cell hit the reference's step caps. */
 int htr_chem_update(const htr_grid *g, const htr_fields *f, double dt, int precision,
                     const htr_mix *m, const htr_diag *diag, void *stream);
 

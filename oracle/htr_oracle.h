@@ -249,6 +249,23 @@ void   o_rosenbrock_testprob(double *x, double dtTry, double DelT, int32_t *nste
 /* one cell's implicit chemistry: x[nEq] in/out, temperature in/out */
 int    o_chem_cell(const o_mix *m, double *x, const double *Conserved_t_old,
                    double *temperature, double dt);
+/* Jacobian of the Rosenbrock step: the reference's central differences
+ * (math_utils.hpp:249-270) or the analytic derivative of the same
+ * right-hand side (o_chem_jacobian_analytic; no reference counterpart) */
+#define O_JAC_FD        0
+#define O_JAC_ANALYTIC  1
+/* o_chem_cell with a Jacobian mode and a step tolerance (the reference: FD,
+ * 1e-10; a tighter tolerance gives the converged solution the precision
+ * paths are measured against) */
+int    o_chem_cell_ex(const o_mix *m, double *x, const double *Conserved_t_old,
+                      double *temperature, double dt, int jacobian, double tol);
+void   o_chem_update_ex(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps,
+                        int jacobian, double tol);
+/* the Jacobian of the chemistry rhs at x (FD: the reference's, with its
+ * temperature-guess carry; analytic: after one rhs evaluation at x) */
+void   o_chem_jacobian(const o_mix *m, const double *x, const double *Conserved_t_old,
+                       double *temperature, int jacobian, double *J);
+int    o_chem_jacobian_analytic(const o_mix *m, const double *x, double Tn, double *J);
 
 /* ---- test hooks: internal steps of the flux reconstruction -------------- */
 /* Roe state (ComputeRoeAverages, src/prometeo_rhs.inl:200-279) of the left and

@@ -496,6 +496,136 @@ void o_GetProductionRates(const o_mix *m, double *w, double rhon, double Pn, dou
    for (int i = 0; i < m->nSpec; i++) w[i] *= m->sp[i].W*m->iwiRef;
 }
 
+/* Analytic Jacobian of the implicit-chemistry right-hand side
+ * (ImplicitSolver::rhs, src/prometeo_chem.hpp:66-88: r_i = w_i(x) + Ct_old_i
+ * for the species, r = Ct_old for momentum and energy) at the conserved state
+ * x[nEq] and its converged temperature Tn.  The reference forms this matrix
+ * by central differences (src/Utils/math_utils.hpp:249-270); this is the exact
+ * derivative of the same functions (MultiComponent.inl:187-210, 371-417,
+ * Reaction.inl:34-228 without fall-off), by the chain rule:
+ *   C_k = rho Y_k rhoRef / W_k (Y clamped to [1e-60, 1] as the rhs does; a
+ *         clamped Y_k moves only with rho),
+ *   ctot = P / (R T) = sum_k C_k, and log T + log(R/P) = -log ctot,
+ *   dT/dx_j from e(x) = sum_k Y_k e_k(T): dT = (de - sum_k e_k dY_k) / cv,
+ *   dKf/dT = Kf (n/T + (E/R)/T^2), d(G_k/RT)/dT = -(H_k/RT)/T,
+ *   Kb = Kf exp(sum nu G - sum nu log ctot).
+ * J is column-major n x n (n = nSpec + 4), rows nSpec.. are zero.
+ * Returns 1 for a mixture with fall-off reactions (not differentiated), 0. */
+int o_chem_jacobian_analytic(const o_mix *m, const double *x, double Tn, double *J)
+{
+   const int ns = m->nSpec, n = ns + 4;
+   memset(J, 0, sizeof(double)*n*n);
+   if (m->constprop) return 0;
+   for (int k = 0; k < m->nReac; k++) if (m->rc[k].kind == O_REAC_FALLOFF) return 1;
+   double rho = 0.0;
+   for (int i = 0; i < ns; i++) rho += x[i];
+   const double irho = 1.0/rho;
+   double Y[O_NSP_MAX], ek[O_NSP_MAX], C[O_NSP_MAX], G[O_NSP_MAX], dG[O_NSP_MAX];
+   int fr[O_NSP_MAX];
+   for (int i = 0; i < ns; i++) {
+      const double y = irho*x[i];
+      Y[i] = dmin(dmax(y, 1e-60), 1.0);
+      fr[i] = (Y[i] == y);
+   }
+   double k2 = 0.0, u[3];
+   for (int d = 0; d < 3; d++) { u[d] = x[ns+d]*irho; k2 += u[d]*u[d]; }
+   const double e = x[ns+3]*irho - 0.5*k2;
+   const double T = Tn*m->TRef, rr = rho*m->rhoRef;
+   double cv = 0.0, sfe = 0.0;
+   for (int i = 0; i < ns; i++) {
+      const o_spec *s = &m->sp[i];
+      const double h = sp_enthalpy(s, T);
+      ek[i] = (h - O_RGAS*T/s->W)*m->ieRef;
+      cv += Y[i]*(sp_cp(s, T) - O_RGAS/s->W)*m->iCpRef;
+      if (fr[i]) sfe += ek[i]*Y[i];
+      C[i] = Y[i]*rr/s->W;
+      G[i] = sp_free_enthalpy(s, T);
+      dG[i] = -(h*s->W/O_RGAS)/(T*T);
+   }
+   /* column vectors: dT/dx_j (nondimensional T) and dctot/dx_j */
+   double dT[O_NEQ_MAX], dct[O_NEQ_MAX];
+   double clampW = 0.0;
+   for (int i = 0; i < ns; i++) if (!fr[i]) clampW += Y[i]/m->sp[i].W;
+   for (int j = 0; j < ns; j++) {
+      dT[j] = ((0.5*k2 - e) - ((fr[j] ? ek[j] : 0.0) - sfe))*irho/cv;
+      dct[j] = m->rhoRef*((fr[j] ? 1.0/m->sp[j].W : 0.0) + clampW);
+   }
+   for (int d = 0; d < 3; d++) { dT[ns+d] = -u[d]*irho/cv; dct[ns+d] = 0.0; }
+   dT[ns+3] = irho/cv; dct[ns+3] = 0.0;
+   double ctot = 0.0;
+   for (int i = 0; i < ns; i++) ctot += C[i];
+   /* per reaction: derivatives of its rate of progress q with respect to T
+    * (dimensional), ctot and the concentrations of its species */
+   for (int k = 0; k < m->nReac; k++) {
+      const o_reac *r = &m->rc[k];
+      const double Kf = rate_coeff(&r->arr, T);
+      double dlKf = 0.0;
+      if (r->arr.n != 0.0) dlKf += (double)r->arr.n/T;
+      if (fabs((double)r->arr.EovR) > 1e-5) dlKf += (double)r->arr.EovR/(T*T);
+      double a = 1.0;
+      for (int i = 0; i < r->Ne; i++) a *= pow(C[r->ed[i].ind], (double)r->ed[i].nu);
+      double Kb = 0.0, b = 1.0, sumNu = 0.0, dlKb = 0.0;
+      if (r->has_backward) {
+         double sumNuG = 0.0, sdG = 0.0;
+         for (int i = 0; i < r->Ne; i++) {
+            sumNu -= r->ed[i].nu; sumNuG -= r->ed[i].nu*G[r->ed[i].ind]; sdG -= r->ed[i].nu*dG[r->ed[i].ind];
+         }
+         for (int i = 0; i < r->Np; i++) {
+            sumNu += r->pd[i].nu; sumNuG += r->pd[i].nu*G[r->pd[i].ind]; sdG += r->pd[i].nu*dG[r->pd[i].ind];
+         }
+         Kb = Kf*exp(sumNuG - sumNu*log(ctot));
+         dlKb = dlKf + sdG;
+         for (int i = 0; i < r->Np; i++) b *= pow(C[r->pd[i].ind], (double)r->pd[i].nu);
+      }
+      const double D = Kf*a - Kb*b;
+      double c = 1.0;
+      const int tb = (r->kind == O_REAC_THIRDB && r->Ntb > 0);
+      if (tb) {
+         c = ctot;
+         for (int i = 0; i < r->Ntb; i++) c += C[r->tb[i].ind]*(r->tb[i].eff - 1.0);
+      }
+      const double QT = c*(Kf*dlKf*a - Kb*dlKb*b)*m->TRef;     /* per nondimensional T */
+      const double Qc = c*Kb*b*sumNu/ctot + (tb ? D : 0.0);
+      /* dq/dC_s for the species of the reaction, as (species, value) pairs */
+      int sid[3*O_NSP_MAX];
+      double sq[3*O_NSP_MAX];
+      int nq = 0;
+      for (int i = 0; i < r->Ne; i++) {
+         double p = (double)r->ed[i].nu*pow(C[r->ed[i].ind], (double)r->ed[i].nu - 1.0);
+         for (int l = 0; l < r->Ne; l++) if (l != i) p *= pow(C[r->ed[l].ind], (double)r->ed[l].nu);
+         sid[nq] = r->ed[i].ind; sq[nq++] = c*Kf*p;
+      }
+      if (r->has_backward)
+         for (int i = 0; i < r->Np; i++) {
+            double p = (double)r->pd[i].nu*pow(C[r->pd[i].ind], (double)r->pd[i].nu - 1.0);
+            for (int l = 0; l < r->Np; l++) if (l != i) p *= pow(C[r->pd[l].ind], (double)r->pd[l].nu);
+            sid[nq] = r->pd[i].ind; sq[nq++] = -c*Kb*p;
+         }
+      if (tb)
+         for (int i = 0; i < r->Ntb; i++) { sid[nq] = r->tb[i].ind; sq[nq++] = D*(r->tb[i].eff - 1.0); }
+      /* row contributions nu_i * dq/dx_j to the species of the reaction */
+      for (int side = 0; side < 2; side++) {
+         const int cnt = side ? r->Np : r->Ne;
+         for (int ii = 0; ii < cnt; ii++) {
+            const int row = side ? r->pd[ii].ind : r->ed[ii].ind;
+            const double nu = side ? (double)r->pd[ii].nu : -(double)r->ed[ii].nu;
+            for (int j = 0; j < n; j++) J[row + j*n] += nu*(QT*dT[j] + Qc*dct[j]);
+            for (int l = 0; l < nq; l++) {
+               const int s = sid[l];
+               const double f = nu*sq[l]*m->rhoRef/m->sp[s].W;
+               if (fr[s]) J[row + s*n] += f;
+               else for (int j = 0; j < ns; j++) J[row + j*n] += f*Y[s];
+            }
+         }
+      }
+   }
+   for (int i = 0; i < ns; i++) {
+      const double sc = m->sp[i].W*m->iwiRef;
+      for (int j = 0; j < n; j++) J[i + j*n] *= sc;
+   }
+   return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Builders                                                                   */
 /* ------------------------------------------------------------------------- */

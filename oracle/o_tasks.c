@@ -334,10 +334,14 @@ void o_lu_solve(int n, const double *Ain, double *b)
 
 /* --------------------------- Rosenbrock solver ---------------------------- */
 typedef void (*rhs_fn)(void *ctx, double *r, const double *x);
+/* Jacobian at x after rhs(x) (analytic mode) */
+typedef void (*jac_fn)(void *ctx, double *J, const double *x);
 
 typedef struct {
    int n;
    rhs_fn rhs;
+   jac_fn jac;                 /* 0: the reference's central differences */
+   double tol;                 /* step tolerance (the reference's 1e-10) */
    void *ctx;
    double err[O_NEQ_MAX], dx[O_NEQ_MAX], xsav[O_NEQ_MAX], dxsav[O_NEQ_MAX];
    double Jac[O_NEQ_MAX*O_NEQ_MAX], LU[O_NEQ_MAX*O_NEQ_MAX];
@@ -434,7 +438,7 @@ static int rb_itercheck(rb_t *s, double *dt, double *dtNext, int finish)
 {
    double errmax = 0.0;
    for (int i = 0; i < s->n; i++) errmax = dmax(errmax, fabs(s->err[i]));
-   errmax *= (1.0/RB_TOL);
+   errmax *= (1.0/s->tol);
    if ((errmax < 1.0) || finish) {
       if (errmax > RB_ERRCON) *dtNext = RB_SAFETY*(*dt)*pow(errmax, RB_PGROW);
       else                    *dtNext = RB_GROW*(*dt);
@@ -466,8 +470,15 @@ static int rb_solve(rb_t *s, double *x, double dtTry, double DelT)
    for (int st = 0; st < RB_MAXSTP; st++) {
       const double t0 = time;
       double dtNext = 0.0;
-      rb_jacobian(s, x);
-      s->rhs(s->ctx, s->dx, x);
+      if (s->jac) {
+         /* analytic Jacobian (not the reference's order: the right-hand side
+          * first, so that the Jacobian sees the converged temperature) */
+         s->rhs(s->ctx, s->dx, x);
+         s->jac(s->ctx, s->Jac, x);
+      } else {
+         rb_jacobian(s, x);
+         s->rhs(s->ctx, s->dx, x);
+      }
       memcpy(s->xsav, x, sizeof(double)*s->n);
       memcpy(s->dxsav, s->dx, sizeof(double)*s->n);
       int jtry;
@@ -493,7 +504,7 @@ static void testprob_rhs(void *ctx, double *r, const double *x)
 void o_rosenbrock_testprob(double *x, double dtTry, double DelT, int32_t *nsteps)
 {
    rb_t *s = (rb_t*)calloc(1, sizeof(rb_t));
-   s->n = 3; s->rhs = testprob_rhs; s->ctx = 0;
+   s->n = 3; s->rhs = testprob_rhs; s->ctx = 0; s->tol = RB_TOL;
    const int st = rb_solve(s, x, dtTry, DelT);
    if (nsteps) *nsteps = st;
    free(s);
@@ -531,17 +542,43 @@ static void chem_rhs(void *vctx, double *r, const double *x)
    for (int i = ns; i < ns+4; i++) r[i] = c->Ct_old[i];
 }
 
-int o_chem_cell(const o_mix *m, double *x, const double *Ct_old, double *temperature, double dt)
+static void chem_jac(void *vctx, double *J, const double *x)
+{
+   chem_ctx *c = (chem_ctx*)vctx;
+   o_chem_jacobian_analytic(c->m, x, *c->T, J);
+}
+
+int o_chem_cell_ex(const o_mix *m, double *x, const double *Ct_old, double *temperature, double dt,
+                   int jacobian, double tol)
 {
    rb_t *s = (rb_t*)calloc(1, sizeof(rb_t));
    chem_ctx c = { m, Ct_old, temperature };
    s->n = m->nSpec + 4; s->rhs = chem_rhs; s->ctx = &c;
+   s->jac = (jacobian == O_JAC_ANALYTIC) ? chem_jac : 0;
+   s->tol = tol;
    const int st = rb_solve(s, x, dt, dt);
    free(s);
    return st;
 }
 
-void o_chem_update(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps)
+int o_chem_cell(const o_mix *m, double *x, const double *Ct_old, double *temperature, double dt)
+{ return o_chem_cell_ex(m, x, Ct_old, temperature, dt, O_JAC_FD, RB_TOL); }
+
+void o_chem_jacobian(const o_mix *m, const double *x0, const double *Ct_old, double *temperature,
+                     int jacobian, double *J)
+{
+   rb_t *s = (rb_t*)calloc(1, sizeof(rb_t));
+   chem_ctx c = { m, Ct_old, temperature };
+   double x[O_NEQ_MAX], r[O_NEQ_MAX];
+   s->n = m->nSpec + 4; s->rhs = chem_rhs; s->ctx = &c;
+   memcpy(x, x0, sizeof(double)*s->n);
+   if (jacobian == O_JAC_ANALYTIC) { chem_rhs(&c, r, x); o_chem_jacobian_analytic(m, x, *temperature, J); }
+   else { rb_jacobian(s, x); memcpy(J, s->Jac, sizeof(double)*s->n*s->n); }
+   free(s);
+}
+
+void o_chem_update_ex(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps,
+                      int jacobian, double tol)
 {
    const long N = NN(g);
    const int neq = m->nSpec + 4;
@@ -557,12 +594,15 @@ void o_chem_update(const o_grid *g, const o_fields *f, double dt, const o_mix *m
                x[s] = f->Conserved[s*N + p];
             }
             double T = f->temperature[p];
-            const int st = o_chem_cell(m, x, old, &T, dt);
+            const int st = o_chem_cell_ex(m, x, old, &T, dt, jacobian, tol);
             for (int s = 0; s < neq; s++) f->Conserved[s*N + p] = x[s];
             f->temperature[p] = T;
             if (nsteps) nsteps[p] = st;
          }
 }
+
+void o_chem_update(const o_grid *g, const o_fields *f, double dt, const o_mix *m, int32_t *nsteps)
+{ o_chem_update_ex(g, f, dt, m, nsteps, O_JAC_FD, RB_TOL); }
 
 /* ------------------------------- RK update -------------------------------- */
 void o_rk_update(const o_grid *g, const o_fields *f, int nEq, int stage, double dt)

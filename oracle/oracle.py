@@ -130,6 +130,9 @@ def _declare(L):
         "o_lu_solve": (None, [i, _dp, _dp]),
         "o_rosenbrock_testprob": (None, [_dp, d, d, C.POINTER(C.c_int32)]),
         "o_chem_cell": (i, [vp, _dp, _dp, _dp, d]),
+        "o_chem_cell_ex": (i, [vp, _dp, _dp, _dp, d, i, d]),
+        "o_chem_update_ex": (None, [C.POINTER(OGrid), C.POINTER(OFields), d, vp, C.POINTER(C.c_int32), i, d]),
+        "o_chem_jacobian": (None, [vp, _dp, _dp, _dp, i, _dp]),
         "o_test_roe": (None, [vp, _dp, _dp, _dp]),
         "o_test_char": (None, [vp, i, _dp, _dp, _dp, i]),
         "o_test_face_flux": (None, [C.POINTER(OGrid), C.POINTER(OFields), i, i, i, i, i, d, d, d, vp, _dp,
@@ -384,9 +387,18 @@ def chem_source(g, f, m):
     lib().o_chem_source(C.byref(g.c()), C.byref(f.c()), m.ptr)
 
 
-def chem_update(g, f, dt, m, nsteps=None):
+JAC_FD, JAC_ANALYTIC = 0, 1
+
+
+def chem_update(g, f, dt, m, nsteps=None, jacobian=JAC_FD, tol=1e-10):
+    """UpdateChemistryTask over the rect; jacobian JAC_FD is the reference's
+    central-difference Jacobian, JAC_ANALYTIC its exact derivative; tol the
+    Rosenbrock step tolerance (the reference's 1e-10)."""
     sp = nsteps.ctypes.data_as(C.POINTER(C.c_int32)) if nsteps is not None else None
-    lib().o_chem_update(C.byref(g.c()), C.byref(f.c()), dt, m.ptr, sp)
+    if jacobian == JAC_FD and tol == 1e-10:
+        lib().o_chem_update(C.byref(g.c()), C.byref(f.c()), dt, m.ptr, sp)
+    else:
+        lib().o_chem_update_ex(C.byref(g.c()), C.byref(f.c()), dt, m.ptr, sp, int(jacobian), float(tol))
 
 
 def max_spectral_radius(g, f, m):
@@ -443,11 +455,22 @@ def rosenbrock_testprob(x0, dtTry, DelT):
     return x, st.value
 
 
-def chem_cell(m, x, ct_old, T, dt):
+def chem_cell(m, x, ct_old, T, dt, jacobian=JAC_FD, tol=1e-10):
     x = _arr(x).copy()
     T_ = C.c_double(T)
-    st = lib().o_chem_cell(m.ptr, dptr(x), dptr(_arr(ct_old)), C.byref(T_), dt)
+    st = lib().o_chem_cell_ex(m.ptr, dptr(x), dptr(_arr(ct_old)), C.byref(T_), dt, int(jacobian), float(tol))
     return x, T_.value, st
+
+
+def chem_jacobian(m, x, ct_old, T, jacobian=JAC_FD):
+    """The Rosenbrock Jacobian of the chemistry right-hand side at x (column-
+    major nEq x nEq, returned as a [row, column] array) and the temperature
+    the evaluation leaves."""
+    n = m.nSpec + 4
+    J = np.zeros(n * n)
+    T_ = C.c_double(T)
+    lib().o_chem_jacobian(m.ptr, dptr(_arr(x).copy()), dptr(_arr(ct_old)), C.byref(T_), int(jacobian), dptr(J))
+    return J.reshape(n, n).T.copy(), T_.value
 
 
 def teno_reconstruct(op, side, y6, nType=0):

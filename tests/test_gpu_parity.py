@@ -252,15 +252,17 @@ def test_chem_source_and_cfl(air16):
 
 
 def chem_case(st, meta, n, om, dt, precision):
-    """One implicit chemistry update on both sides.  Returns the per-species
-    errors of rho Y_i, absolute (normalised units, the units of the solver's
-    1e-10 step tolerance) and relative to max |rho Y_i|, the temperature error
-    and the step counts."""
+    """One implicit chemistry update on both sides (the oracle with the same
+    Jacobian kind as the GPU path).  Returns the per-species errors of rho Y_i,
+    absolute (normalised units, the units of the solver's 1e-10 step
+    tolerance) and relative to max |rho Y_i|, the temperature error and the
+    step counts."""
     import htr_amd as H
     g, f = oracle_fields(st, om.nSpec, n)
     f["Conserved_t"] = 0.0
     ns_o = np.zeros(f["rho"].shape, np.int32)
-    O.chem_update(g, f, dt, om, ns_o)
+    jac = O.JAC_ANALYTIC if precision & H.HTR_CHEM_JAC_ANALYTIC else O.JAC_FD
+    O.chem_update(g, f, dt, om, ns_o, jacobian=jac)
     fs = to_gpu(st, om.nSpec, n, nsteps=True)
     fs["Conserved_t"] = 0.0
     H.update_chemistry(fs, dt, gpu_mix(meta), precision=precision, diag=True)
@@ -306,7 +308,10 @@ def air16s():
     return st, meta, n, om
 
 
-@pytest.mark.parametrize("precision,label", [(0, "FP64"), (1, "FP32+IR")])
+CHEM_PRECISIONS = [(0, "FP64"), (1, "FP32+IR"), (0x10, "FP64 analytic J"), (0x11, "FP32+IR analytic J")]
+
+
+@pytest.mark.parametrize("precision,label", CHEM_PRECISIONS)
 def test_chem_update_air(air16s, precision, label):
     st, meta, n, om = air16s
     say("case air16 per-seed implicit chemistry", label)
@@ -314,7 +319,7 @@ def test_chem_update_air(air16s, precision, label):
     check_chem(label, *chem_case(st, meta, n, om, dt, precision), names=["N2", "O2", "NO", "N", "O"])
 
 
-@pytest.mark.parametrize("precision,label", [(0, "FP64"), (1, "FP32+IR")])
+@pytest.mark.parametrize("precision,label", CHEM_PRECISIONS)
 def test_chem_update_30_species(precision, label):
     """Batched implicit chemistry with the 30-species synthetic mechanism
     (34 x 34 Jacobians) on 4^3 cells."""
