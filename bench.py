@@ -97,6 +97,8 @@ def parse():
     ap.add_argument("--chem-tune", action="store_true", help="run the launch-shape autotuner first")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--overlap", default="off", choices=["on", "off"],
+                    help="hybrid full RHS with each direction's TENO passes on a side stream (on) or one stream")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB FP64 tensor 5 times (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -403,13 +405,12 @@ def measure_rhs(H, a, workload, composition, traffic_table=False):
           file=sys.stderr, flush=True)
     counts = face_counts(H, fs, mix, scheme, dt, n, grid)
     print("TENO faces per direction %s" % counts, file=sys.stderr, flush=True)
+    H.rhs_overlap(a.overlap == "on")
     H.workspace_reserve(fs, mix, grid=grid)
     print("timing %d RHS steps" % (a.warmup + a.steps), file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         run_rhs(H, fs, mix, scheme, dt, grid)
     torch.cuda.synchronize()
-    H.profile(True)
-    H.profile_report()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -418,10 +419,24 @@ def measure_rhs(H, a, workload, composition, traffic_table=False):
         run_rhs(H, fs, mix, scheme, dt, grid)
     t1.record()
     torch.cuda.synchronize()
-    rep = H.profile_report()
-    H.profile(False)
     total_ms = t0.elapsed_time(t1)
     ms_step = total_ms / a.steps
+    # per-kernel times (HIP events around every launch) from a second pass
+    # with every launch on one stream: with the overlap, kernels share the
+    # GPU and an event pair would time the sharing, not the kernel
+    H.rhs_overlap(False)
+    torch.cuda.synchronize()
+    H.profile(True)
+    H.profile_report()
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p0.record()
+    for _ in range(a.steps):
+        run_rhs(H, fs, mix, scheme, dt, grid)
+    p1.record()
+    torch.cuda.synchronize()
+    rep = H.profile_report()
+    H.profile(False)
+    ms_one_stream = p0.elapsed_time(p1) / a.steps
     per_launch = {k: ms / cnt for k, (cnt, ms) in rep.items()}
     per_step = {k: ms / a.steps for k, (cnt, ms) in rep.items()}
     dom = max(per_step, key=per_step.get)
@@ -450,6 +465,7 @@ def measure_rhs(H, a, workload, composition, traffic_table=False):
         roof["algorithmic_bytes"] = bpu * units
     faces = sum(nfaces_dir)
     out = {"value": cells / (ms_step * 1e-3), "ms_per_step": ms_step, "cells": cells,
+           "rhs_overlap": a.overlap == "on", "ms_per_step_one_stream_profiled": ms_one_stream,
            "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
            "teno_face_frac": sum(c[0] for c in counts.values()) / faces,
            "limiter_faces": {d: c[1] for d, c in counts.items()},
@@ -503,7 +519,8 @@ def main():
                    "terms": "chem source + Euler ZYX + diffusion ZYX", "seed": a.seed,
                    "layout": a.layout if a.workload == "air" else "soa"},
     }
-    for k in ("kernel_ms_per_step", "teno_face_frac", "limiter_faces", "rhs_roofline", "roofline"):
+    for k in ("kernel_ms_per_step", "teno_face_frac", "limiter_faces", "rhs_roofline", "roofline", "rhs_overlap",
+              "ms_per_step_one_stream_profiled"):
         result[k] = main_r[k]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if default_workload and not a.no_extras:

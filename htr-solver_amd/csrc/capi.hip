@@ -258,6 +258,7 @@ static int upload(htr_mix *m)
    int rc = ensure_device();
    if (rc) return rc;
    if (!m->host.constprop) build_species_reactions(m->host);
+   for (int i = 0; i < m->host.nSpec; i++) derive_species(m->host.sp[i]);
    if (!m->dev && hipMalloc(&m->dev, sizeof(DMix)) != hipSuccess)
       return set_error(HTR_E_DEVICE, "mixture allocation failed");
    if (hipMemcpy(m->dev, &m->host, sizeof(DMix), hipMemcpyHostToDevice) != hipSuccess)
@@ -424,6 +425,74 @@ static int prep(const htr_grid *g, const htr_fields *f, const htr_mix *m, htr_fi
 static int need(const htr_view &v, const char *name)
 {
    if (!v.ptr) return set_error(HTR_E_FIELD, "field %s is required", name);
+   return HTR_OK;
+}
+
+// Side streams of the full RHS with overlap, per (device, caller stream):
+// one stream per direction for that direction's TENO passes (each stream has
+// its own workspace, so the three face buffers live at once) and the events
+// of the hand-overs.  Default priority, non-blocking (ordered by the events
+// only).
+struct RhsAux {
+   hipStream_t ts[3] = {};
+   hipEvent_t  h = nullptr;           // caller stream: total enthalpy formed
+   hipEvent_t  teno[3] = {};          // side stream d: TENO faces of direction d formed
+   hipEvent_t  used[3] = {};          // caller stream: the walks of direction d have read them
+};
+static std::map<std::pair<int, uintptr_t>, RhsAux*> g_aux;
+static int g_overlap = 0;
+
+static RhsAux *rhs_aux(hipStream_t s)
+{
+   int dev = 0;
+   (void)hipGetDevice(&dev);
+   std::lock_guard<std::mutex> lk(g_mu);
+   RhsAux *&a = g_aux[std::make_pair(dev, (uintptr_t)s)];
+   if (a) return a;
+   RhsAux *n = new RhsAux();
+   bool ok = hipEventCreateWithFlags(&n->h, hipEventDisableTiming) == hipSuccess;
+   for (int d = 0; d < 3 && ok; d++)
+      ok = hipStreamCreateWithFlags(&n->ts[d], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&n->teno[d], hipEventDisableTiming) == hipSuccess &&
+           hipEventCreateWithFlags(&n->used[d], hipEventDisableTiming) == hipSuccess;
+   if (!ok) {
+      set_error(HTR_E_DEVICE, "side stream creation failed");
+      return nullptr;
+   }
+   a = n;
+   return a;
+}
+
+// The hybrid Euler sweeps with each direction's TENO passes on a side stream:
+// total enthalpy (caller stream), then the TENO passes of z, y, x on their
+// side streams, overlapping the chemistry source and the earlier directions'
+// KG walks on the caller stream; each walk waits for its direction's TENO
+// faces.  Conserved_t receives its terms in the same order as without
+// overlap (source, Euler z, y, x), so the result is the same to the bit.
+template<class MX>
+static int euler_overlapped(const htr_fields &fr, const htr_mix *m, const Geo &geo, double RK0, double RK1,
+                            const DtArg &dt, const htr_view &dec, int chem_source, hipStream_t s)
+{
+   RhsAux *aux = rhs_aux(s);
+   if (!aux) return HTR_E_DEVICE;
+   int rc = enthalpy_launch<MX>(fr, geo, s);
+   if (rc) return rc;
+   if (hipEventRecord(aux->h, s) != hipSuccess) return set_error(HTR_E_DEVICE, "event record failed");
+   for (int d = 2; d >= 0; d--) {
+      if (hipStreamWaitEvent(aux->ts[d], aux->h, 0) != hipSuccess ||
+          hipStreamWaitEvent(aux->ts[d], aux->used[d], 0) != hipSuccess)
+         return set_error(HTR_E_DEVICE, "stream wait failed");
+      if ((rc = hybrid_part_dispatch<MX>(d, 0, fr, m->dev, geo, RK0, RK1, dt, dec, aux->ts[d], s, s))) return rc;
+      if (hipEventRecord(aux->teno[d], aux->ts[d]) != hipSuccess) return set_error(HTR_E_DEVICE, "event record failed");
+   }
+   if (chem_source && !m->host.constprop) rc = chem_source_launch<MixAir>(fr, m->dev, geo, s, true);
+   else                                   rc = zero_rhs_launch(fr, geo, m->host.nSpec + 4, s);
+   if (rc) return rc;
+   for (int d = 2; d >= 0; d--) {
+      if (hipStreamWaitEvent(s, aux->teno[d], 0) != hipSuccess) return set_error(HTR_E_DEVICE, "stream wait failed");
+      if ((rc = hybrid_part_dispatch<MX>(d, 1, fr, m->dev, geo, RK0, RK1, dt, dec, aux->ts[d], s, s))) return rc;
+      if (hipEventRecord(aux->used[d], s) != hipSuccess) return set_error(HTR_E_DEVICE, "event record failed");
+   }
    return HTR_OK;
 }
 
@@ -883,7 +952,25 @@ int htr_workspace_reserve(const htr_grid *g, const htr_fields *f, const htr_mix 
    Workspace *w = workspace((hipStream_t)stream);
    if (!w) return HTR_E_DEVICE;
    if ((rc = workspace_faces(*w, need.faces, need.list, need.teno))) return rc;
-   return workspace_enthalpy(*w, need.hbuf);
+   if ((rc = workspace_enthalpy(*w, need.hbuf))) return rc;
+   if (g_overlap) {
+      // the side streams of the full RHS, one direction's TENO faces each
+      RhsAux *aux = rhs_aux((hipStream_t)stream);
+      if (!aux) return HTR_E_DEVICE;
+      for (int d = 0; d < 3; d++) {
+         Workspace *a = workspace(aux->ts[d]);
+         if (!a) return HTR_E_DEVICE;
+         if ((rc = workspace_faces(*a, need.faces, need.list, need.teno))) return rc;
+      }
+   }
+   return HTR_OK;
+}
+
+int htr_rhs_overlap(int on)
+{
+   if (on != 0 && on != 1) return set_error(HTR_E_ARG, "overlap flag %d", on);
+   g_overlap = on;
+   return HTR_OK;
 }
 
 static int full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
@@ -901,6 +988,16 @@ static int full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double R
    hipStream_t s = (hipStream_t)stream;
    const MixKind k = kind_of(m);
    if (k != K_CP && k != K_AIR) return set_error(HTR_E_UNSUPPORTED, "no kernel instance for %d species", m->host.nSpec);
+   if (g_overlap && scheme == HTR_EULER_HYBRID && exec_cells(geo) > 0) {
+      rc = (k == K_CP) ? euler_overlapped<MixCP>(fr, m, geo, RK0, RK1, dt, dec, chem_source, s)
+                       : euler_overlapped<MixAir>(fr, m, geo, RK0, RK1, dt, dec, chem_source, s);
+      if (rc) return rc;
+      for (int d = 2; d >= 0; d--) {
+         rc = htr_diffusion_flux(d, g, f, m, stream);
+         if (rc) return rc;
+      }
+      return HTR_OK;
+   }
    if (chem_source && !m->host.constprop) {
       HTR_NEED(pressure); HTR_NEED(temperature);
       rc = chem_source_launch<MixAir>(fr, m->dev, geo, s, true);     // zeroes Conserved_t as it goes

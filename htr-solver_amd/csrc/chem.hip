@@ -139,7 +139,7 @@ struct ChemGroup {
          gsync();
          const double f = e0 - sum_in_order(ta, NS)*m.ieRef;
          if (fabs(f/dfdT) < 1e-8) break;
-         for (int i = lane; i < NS; i += L) tb[i] = Y[i]*(sp_cp(m.sp[i], Tn*m.TRef) - RGAS/m.sp[i].W)*m.iCpRef;
+         for (int i = lane; i < NS; i += L) tb[i] = Y[i]*(sp_cp(m.sp[i], Tn*m.TRef) - m.sp[i].rOvW)*m.iCpRef;
          gsync();
          dfdT = sum_in_order(tb, NS);
          Tn += f/dfdT;
@@ -285,7 +285,7 @@ struct ChemGroup {
          fr[i] = f ? 1.0 : 0.0;
          const double h = sp_enthalpy(sp, Tr);
          ek[i] = (h - RGAS*Tr/sp.W)*m.ieRef;
-         tcv[i] = yc*(sp_cp(sp, Tr) - RGAS/sp.W)*m.iCpRef;
+         tcv[i] = yc*(sp_cp(sp, Tr) - sp.rOvW)*m.iCpRef;
          tse[i] = f ? ek[i]*yc : 0.0;
          tcw[i] = f ? 0.0 : yc/sp.W;
          C[i] = yc*rr/sp.W;

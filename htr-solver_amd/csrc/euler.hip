@@ -1226,6 +1226,52 @@ static int teno_faces(const htr_fields &f, const DMix *mix, const Geo &g, const 
    return launch_teno<MX, DIR, HTR_EULER_TENO, true>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, F, dec, s);
 }
 
+// The KG faces and the fused divergence of a hybrid (or pure KG, ALL) sweep
+// along DIR, the listed TENO faces read back from F
+template<class MX, int DIR, bool ALL>
+static int kg_walks(const htr_fields &f, const Geo &g, const htr_view &hv, const double *F, int64_t nfaces,
+                    const htr_view &dec, hipStream_t s)
+{
+   int rc;
+   if constexpr (DIR == 0) {
+      const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XK_OUT - 1)/XK_OUT;
+      const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
+      if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many rows");
+      HTR_LAUNCH("euler_kg_species_x", (euler_kg_x_kernel<MX, ALL, 0>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
+                 s, f, g, hv, F, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
+      if ((rc = check_launch("euler_kg_x_kernel"))) return rc;
+      HTR_LAUNCH("euler_kg_flow_x", (euler_kg_x_kernel<MX, ALL, 1>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
+                 s, f, g, hv, F, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
+      return check_launch("euler_kg_x_kernel");
+   } else {
+      const int L = walk_segment(DIR, (int64_t)g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B], (int64_t)g.esz[DIR] + 1);
+      const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
+      const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
+      if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many line segments");
+      HTR_LAUNCH(HTR_DIR_NAME("euler_kg_species", DIR), (euler_kg_walk_kernel<MX, DIR, ALL, 0>),
+                 dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, g, hv, F, nfaces, L, (uint32_t)nseg,
+                 (uint32_t)nunits, dec);
+      if ((rc = check_launch("euler_kg_walk_kernel"))) return rc;
+      HTR_LAUNCH(HTR_DIR_NAME("euler_kg_flow", DIR), (euler_kg_walk_kernel<MX, DIR, ALL, 1>),
+                 dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, g, hv, F, nfaces, L, (uint32_t)nseg,
+                 (uint32_t)nunits, dec);
+      return check_launch("euler_kg_walk_kernel");
+   }
+}
+
+// the caller's total-enthalpy array (workspace of stream s) as a view
+static int enthalpy_view(const Geo &g, Workspace &w, htr_view &hv)
+{
+   const int64_t span = ((int64_t)g.bsz[0] - 1)*g.cb[0] + ((int64_t)g.bsz[1] - 1)*g.cb[1] +
+                        ((int64_t)g.bsz[2] - 1)*g.cb[2] + 1;
+   int rc = workspace_enthalpy(w, span);
+   if (rc) return rc;
+   hv = {};
+   hv.ptr = w.hbuf;
+   hv.sx = 8;                                  // one double per base element (k = 1)
+   return HTR_OK;
+}
+
 template<class MX, int DIR, int SCHEME>
 static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
                       const DtArg &dt, const htr_view &dec, hipStream_t s, bool enthalpy_ready)
@@ -1242,12 +1288,8 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
    if (rc) return rc;
    // total enthalpy of every cell of the bounds
    const int64_t nb_cells = (int64_t)g.bsz[0]*g.bsz[1]*g.bsz[2];
-   const int64_t span = ((int64_t)g.bsz[0] - 1)*g.cb[0] + ((int64_t)g.bsz[1] - 1)*g.cb[1] +
-                        ((int64_t)g.bsz[2] - 1)*g.cb[2] + 1;
-   if ((rc = workspace_enthalpy(w, span))) return rc;
-   htr_view hv = {};
-   hv.ptr = w.hbuf;
-   hv.sx = 8;                                  // one double per base element (k = 1)
+   htr_view hv;
+   if ((rc = enthalpy_view(g, w, hv))) return rc;
    if (!enthalpy_ready) {
       HTR_LAUNCH("enthalpy", (enthalpy_kernel<MX>), dim3(grid_for(nb_cells, 256)), dim3(256), 0, s, f, g, hv, nb_cells);
       if ((rc = check_launch("enthalpy_kernel"))) return rc;
@@ -1259,31 +1301,7 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
       if ((rc = teno_faces<MX, DIR>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, w.faces, dec, s))) return rc;
    }
    if constexpr (SCHEME == HTR_EULER_HYBRID || SCHEME == HTR_EULER_KG) {
-      constexpr bool ALL = SCHEME == HTR_EULER_KG;
-      if constexpr (DIR == 0) {
-         const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XK_OUT - 1)/XK_OUT;
-         const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
-         if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many rows");
-         HTR_LAUNCH("euler_kg_species_x", (euler_kg_x_kernel<MX, ALL, 0>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
-                    s, f, g, hv, w.faces, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
-         if ((rc = check_launch("euler_kg_x_kernel"))) return rc;
-         HTR_LAUNCH("euler_kg_flow_x", (euler_kg_x_kernel<MX, ALL, 1>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
-                    s, f, g, hv, w.faces, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
-         return check_launch("euler_kg_x_kernel");
-      } else {
-         const int L = walk_segment(DIR, (int64_t)g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B], (int64_t)g.esz[DIR] + 1);
-         const int64_t nseg = ((int64_t)g.esz[DIR] + 1 + L - 1)/L;
-         const int64_t nunits = nseg*g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B];
-         if (nunits >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many line segments");
-         HTR_LAUNCH(HTR_DIR_NAME("euler_kg_species", DIR), (euler_kg_walk_kernel<MX, DIR, ALL, 0>),
-                    dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, g, hv, w.faces, nfaces, L, (uint32_t)nseg,
-                    (uint32_t)nunits, dec);
-         if ((rc = check_launch("euler_kg_walk_kernel"))) return rc;
-         HTR_LAUNCH(HTR_DIR_NAME("euler_kg_flow", DIR), (euler_kg_walk_kernel<MX, DIR, ALL, 1>),
-                    dim3(grid_for(nunits, 256)), dim3(256), 0, s, f, g, hv, w.faces, nfaces, L, (uint32_t)nseg,
-                    (uint32_t)nunits, dec);
-         return check_launch("euler_kg_walk_kernel");
-      }
+      return kg_walks<MX, DIR, SCHEME == HTR_EULER_KG>(f, g, hv, w.faces, nfaces, dec, s);
    } else {
       if ((rc = launch_teno<MX, DIR, SCHEME, false>(f, mix, g, hv, RK0, RK1, dt, w, nfaces, w.faces, dec, s))) return rc;
       HTR_LAUNCH(HTR_DIR_NAME("euler_divergence", DIR), (face_divergence_kernel<MX::NEQ, DIR>), dim3(grid_for(ncells, 256)),
@@ -1291,6 +1309,48 @@ static int launch_dir(const htr_fields &f, const DMix *mix, const Geo &g, double
       return check_launch("face_divergence_kernel");
    }
 }
+
+// The hybrid sweep along DIR in two parts on two streams (the full RHS with
+// overlap): part 0, the TENO faces into the face buffer of the workspace of
+// stream ts, with the total enthalpy of the workspace of stream hs (formed
+// before); part 1, on stream s, the KG faces and the divergence reading the
+// TENO faces back from the workspace of ts.
+template<class MX, int DIR>
+static int hybrid_part(int part, const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
+                       const DtArg &dt, const htr_view &dec, hipStream_t ts, hipStream_t hs, hipStream_t s)
+{
+   if (exec_cells(g) == 0) return HTR_OK;
+   const int64_t nfaces = Faces<DIR>::count(g);
+   if (nfaces*MX::NEQ >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many faces");
+   Workspace *tw = workspace(ts), *hw = workspace(hs);
+   if (!tw || !hw) return HTR_E_DEVICE;
+   htr_view hv;
+   int rc = enthalpy_view(g, *hw, hv);
+   if (rc) return rc;
+   if (part == 0) {
+      if ((rc = workspace_faces(*tw, nfaces*MX::NEQ, 2*nfaces, teno_doubles<MX>(nfaces)))) return rc;
+      return teno_faces<MX, DIR>(f, mix, g, hv, RK0, RK1, dt, *tw, nfaces, tw->faces, dec, ts);
+   }
+   if (tw->faces_cap < nfaces*MX::NEQ) return set_error(HTR_E_ARG, "euler: TENO part not run");
+   return kg_walks<MX, DIR, false>(f, g, hv, tw->faces, nfaces, dec, s);
+}
+
+template<class MX>
+int hybrid_part_dispatch(int dir, int part, const htr_fields &f, const DMix *mix, const Geo &g, double RK0, double RK1,
+                         const DtArg &dt, const htr_view &dec, hipStream_t ts, hipStream_t hs, hipStream_t s)
+{
+   switch (dir) {
+      case 0: return hybrid_part<MX, 0>(part, f, mix, g, RK0, RK1, dt, dec, ts, hs, s);
+      case 1: return hybrid_part<MX, 1>(part, f, mix, g, RK0, RK1, dt, dec, ts, hs, s);
+      case 2: return hybrid_part<MX, 2>(part, f, mix, g, RK0, RK1, dt, dec, ts, hs, s);
+      default: return set_error(HTR_E_ARG, "euler: bad direction");
+   }
+}
+
+template int hybrid_part_dispatch<MixCP>(int, int, const htr_fields &, const DMix *, const Geo &, double, double,
+                                         const DtArg &, const htr_view &, hipStream_t, hipStream_t, hipStream_t);
+template int hybrid_part_dispatch<MixAir>(int, int, const htr_fields &, const DMix *, const Geo &, double, double,
+                                          const DtArg &, const htr_view &, hipStream_t, hipStream_t, hipStream_t);
 
 // Total enthalpy of every cell of the bounds into the stream's workspace
 // (once per RHS; the three sweeps then skip it)

@@ -46,6 +46,12 @@ template<class MX> int enthalpy_launch(const htr_fields &f, const Geo &g, hipStr
 template<class MX> int euler_dispatch(int dir, int scheme, const htr_fields &f, const DMix *mix, const Geo &g,
                                       double RK0, double RK1, const DtArg &dt, const htr_view &dec, hipStream_t s,
                                       bool enthalpy_ready = false);
+// the hybrid sweep in two parts (full RHS with overlap): part 0, the TENO
+// faces into the workspace of stream ts (total enthalpy from the workspace of
+// hs); part 1, on stream s, the KG faces and the divergence reading them back
+template<class MX> int hybrid_part_dispatch(int dir, int part, const htr_fields &f, const DMix *mix, const Geo &g,
+                                            double RK0, double RK1, const DtArg &dt, const htr_view &dec,
+                                            hipStream_t ts, hipStream_t hs, hipStream_t s);
 template<class MX> int diffusion_dispatch(int dir, const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int properties_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);
 template<class MX> int conserved_launch(const htr_fields &f, const DMix *mix, const Geo &g, hipStream_t s);

@@ -35,7 +35,12 @@ struct DSpec {
    float  TSwitch1, TSwitch2, TMin, TMax;
    int    nPoly;
    int    geom;
-   double cp[3][9];            // [0] high, [1] mid, [2] low range
+   // NASA coefficients per range ([0] high, [1] mid, [2] low), then three
+   // quotients of them the polynomials use, formed once at upload (the same
+   // correctly rounded divisions, so every value is unchanged):
+   // [9] c4/3, [10] c5/6, [11] c6/5
+   double cp[3][12];
+   double rOvW;                // R/W, formed once at upload
    double sigma, kbOveps, mu, alpha, Z298;
 };
 
@@ -84,9 +89,20 @@ HD const double *cp_range(const DSpec &s, double T)
    return (T > (double)s.TSwitch1) ? s.cp[1] : s.cp[2];
 }
 
+// the quotients of DSpec formed at upload
+inline void derive_species(DSpec &s)
+{
+   for (int r = 0; r < 3; r++) {
+      s.cp[r][9] = s.cp[r][4]/3;
+      s.cp[r][10] = s.cp[r][5]/6;
+      s.cp[r][11] = s.cp[r][6]/5;
+   }
+   s.rOvW = RGAS/s.W;
+}
+
 HD double sp_cp(const DSpec &s, double T)
 {
-   const double rOvW = RGAS/s.W;
+   const double rOvW = s.rOvW;
    const double Tinv = 1.0/T;
    const double *c = cp_range(s, T);
    const double poly = c[3] + T*(c[4] + T*(c[5] + T*c[6]));
@@ -99,15 +115,15 @@ HD double sp_free_enthalpy(const DSpec &s, double T)
    const double logT = log(T);
    const double *c = cp_range(s, T);
    const double G = -0.5*c[0]*Tinv*Tinv + c[1]*Tinv*(1.0 + logT) + c[2]*(1.0 - logT) + c[7]*Tinv - c[8];
-   return G - 0.5*T*(c[3] + T*(c[4]/3 + T*(c[5]/6 + 0.1*T*c[6])));
+   return G - 0.5*T*(c[3] + T*(c[9] + T*(c[10] + 0.1*T*c[6])));
 }
 
 HD double sp_enthalpy(const DSpec &s, double T)
 {
-   const double rOvW = RGAS/s.W;
+   const double rOvW = s.rOvW;
    const double Tinv = 1.0/T;
    const double *c = cp_range(s, T);
-   const double E = -c[0]*Tinv + c[1]*log(T) + c[7] + T*(c[2] + T*(c[3]*0.50 + T*(c[4]/3 + T*(c[5]*0.25 + c[6]/5*T))));
+   const double E = -c[0]*Tinv + c[1]*log(T) + c[7] + T*(c[2] + T*(c[3]*0.50 + T*(c[9] + T*(c[5]*0.25 + c[11]*T))));
    return E*rOvW;
 }
 
@@ -253,7 +269,7 @@ struct Mix {
          const double f = e0 - InternalEnergy(m, T, Y);
          if (fabs(f/dfdT) < 1e-8) break;
          dfdT = 0.0;
-         for (int i = 0; i < NS; i++) dfdT += Y[i]*(sp_cp(m.sp[i], T*m.TRef) - RGAS/m.sp[i].W)*m.iCpRef;
+         for (int i = 0; i < NS; i++) dfdT += Y[i]*(sp_cp(m.sp[i], T*m.TRef) - m.sp[i].rOvW)*m.iCpRef;
          T += f/dfdT;
          j++;
       }

@@ -63,7 +63,7 @@ EXPORTS = [
     "htr_profile", "htr_profile_report", "htr_chem_launch_config", "htr_stencil_table",
     "htr_workspace_reserve", "htr_rk_stage", "htr_full_rhs_cfl", "htr_set_bc",
     "htr_max_spectral_radius_noalloc", "htr_nscbc_inflow", "htr_nscbc_outflow", "htr_nscbc_farfield",
-    "htr_max_mach",
+    "htr_max_mach", "htr_rhs_overlap",
 ]
 
 _lib = None
@@ -89,6 +89,7 @@ def _declare(L):
         "htr_init": (i, [i]), "htr_last_error": (C.c_char_p, []), "htr_version": (C.c_char_p, []),
         "htr_profile": (i, [i]), "htr_profile_report": (i, [C.c_char_p, C.c_size_t]),
         "htr_chem_launch_config": (i, [i, i]),
+        "htr_rhs_overlap": (i, [i]),
         "htr_stencil_table": (i, [i, i, i, dp]),
         "htr_workspace_reserve": (i, [G, F, vp, vp]),
         "htr_rk_stage": (i, [G, F, i, d, d, vp, vp, D, vp]),

@@ -25,7 +25,7 @@ __all__ = ["Mixture", "FieldSet", "make_grid", "AIR_SPECIES",
            "chem_launch_config", "chem_autotune", "rk_stage", "full_rhs_cfl", "workspace_reserve", "set_bc",
            "BC_SIDES", "update_using_flux_nscbc_inflow", "update_using_flux_nscbc_outflow",
            "update_using_flux_nscbc_far_field",
-           "calculate_max_mach_number"]
+           "calculate_max_mach_number", "rhs_overlap"]
 
 
 def stream_ptr(stream=None):
@@ -285,6 +285,14 @@ def rk_update(fs, nEq, stage, dt, grid=None, stream=None):
 
 def rk_init(fs, nEq, grid=None, stream=None):
     check(lib().htr_rk_init(_g(fs, grid), C.byref(fs.c()), nEq, stream_ptr(stream)))
+
+
+def rhs_overlap(on):
+    """Scheduling of the hybrid full RHS: True runs each direction's TENO
+    passes on a library side stream, overlapping the chemistry source and the
+    earlier directions' KG walks (same result to the bit); False (the default)
+    runs every launch on the caller's stream."""
+    check(lib().htr_rhs_overlap(1 if on else 0))
 
 
 def full_rhs(fs, scheme, RK_coeffs, deltaTime, chem_source, mix, grid=None, stream=None):

@@ -328,6 +328,17 @@ int htr_full_rhs(const htr_grid *g, const htr_fields *f, int scheme, double RK0,
 int htr_full_rhs_cfl(const htr_grid *g, const htr_fields *f, int scheme, double RK0, double RK1,
                      double cfl, double scale, const double *d_radius, int chem_source, const htr_mix *m,
                      void *stream);
+/* Scheduling of the hybrid full RHS (no reference counterpart; default 0):
+ * with on = 1 each direction's TENO passes run on a side stream of the
+ * library (one per direction and caller stream, created at first use or by
+ * htr_workspace_reserve, each with its own workspace) while the caller's
+ * stream runs the chemistry source and the earlier directions' KG walks;
+ * events order them, and Conserved_t receives its terms in the same order,
+ * so the result is the same to the bit.  on = 0: every launch on the caller's
+ * stream.  Applies to later calls.  The side streams gain only where the TENO
+ * component kernel leaves room on the CUs for the walks' waves; at its full
+ * persistent grid (3 blocks per CU) one stream is faster. */
+int htr_rhs_overlap(int on);
 
 #ifdef __cplusplus
 }

@@ -236,6 +236,11 @@ inline double atomicAdd(double *p, double v)
 }
 // wave-uniform values only (the library reads a uniform value through it)
 template<class T> inline T __builtin_amdgcn_readfirstlane(T v) { return v; }
+// the value of v in lane l of the wave (l uniform): an exchange like __shfl
+inline int __builtin_amdgcn_readlane(int v, int l)
+{
+   return ::hipemu::unbits<int>(::hipemu::exchange(::hipemu::bits(v), l & 63));
+}
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 inline void __builtin_amdgcn_wave_barrier() { ::hipemu::t_wave->bar.wait(); }
 inline void __builtin_amdgcn_sched_barrier(int) {}
@@ -265,4 +270,10 @@ inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (hipEvent_t)malloc(1); re
 inline hipError_t hipEventDestroy(hipEvent_t e) { free(e); return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+// launches run to completion in call order, so streams are distinct handles
+// (separate library workspaces) and the waits between them hold already
+enum { hipStreamNonBlocking = 1, hipEventDisableTiming = 2 };
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (hipStream_t)malloc(1); return hipSuccess; }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (hipEvent_t)malloc(1); return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }

@@ -114,17 +114,21 @@ def main():
             H.update_using_diffusion_flux(fs, d, gpu_mix(meta))
             report("diffusion_d%d" % d, fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
     if "rhs" in which:
-        # the full RHS runs the fused x/y Kennedy-Gruber pass: whole box,
-        # a sub-rect, pure KG, a box with staggered walls in y, and the TGV
+        # the full RHS (hybrid: with each direction's TENO passes on a side
+        # stream of the library, and with every launch on one stream): whole
+        # box, a sub-rect, pure KG, a box with staggered walls in y, the TGV
         dt = dt_cfl(st, meta, n)
-        for label, scheme, lo, hi in (("full_rhs", O.SCHEME_HYBRID, (0, 0, 0), (7, 7, 7)),
-                                      ("full_rhs_subrect", O.SCHEME_HYBRID, (1, 2, 3), (6, 7, 5)),
-                                      ("full_rhs_kg", O.SCHEME_KG, (0, 0, 0), (7, 7, 7))):
+        for label, scheme, lo, hi, ov in (("full_rhs", O.SCHEME_HYBRID, (0, 0, 0), (7, 7, 7), True),
+                                          ("full_rhs_one_stream", O.SCHEME_HYBRID, (0, 0, 0), (7, 7, 7), False),
+                                          ("full_rhs_subrect", O.SCHEME_HYBRID, (1, 2, 3), (6, 7, 5), True),
+                                          ("full_rhs_kg", O.SCHEME_KG, (0, 0, 0), (7, 7, 7), True)):
             g, f = oracle_fields(st, 5, n)
             g = g.with_rect(lo, hi)
             O.full_rhs(g, f, scheme, 1.0, 0.0, dt, 1, om)
             fs = to_host_fs(st, 5, n)
+            H.rhs_overlap(ov)
             H.full_rhs(fs, scheme, (1.0, 0.0), dt, True, gpu_mix(meta), grid=fs.grid(lo, hi))
+            H.rhs_overlap(False)
             report(label, fs["Conserved_t"].numpy(), f["Conserved_t"], 512)
         stw, metaw, nw = wall_state(8, 6, 8)
         stw, omw = complete_state(stw, metaw, nw)

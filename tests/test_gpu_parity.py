@@ -347,6 +347,31 @@ def test_full_rhs_air(air16, layout):
     assert max(e) <= TOL
 
 
+def test_full_rhs_overlap_same_as_one_stream(air16):
+    """The hybrid full RHS with each direction's TENO passes on side streams
+    gives the one-stream result to the bit, on three calls in a row (the side
+    streams' face buffers are reused only after the walks have read them)."""
+    import torch
+    import htr_amd as H
+    st, meta, n, om = air16
+    say("case air16 full RHS, side streams vs one stream")
+    dt = dt_cfl(st, meta, n)
+    out = {}
+    for ov in (False, True):
+        H.rhs_overlap(ov)
+        fs = to_gpu(st, 5, n)
+        res = []
+        for _ in range(3):
+            H.full_rhs(fs, 0, (1.0, 0.0), dt, True, gpu_mix(meta))
+            res.append(host(fs["Conserved_t"]).copy())
+        torch.cuda.synchronize()
+        out[ov] = res
+    H.rhs_overlap(False)
+    for k in range(3):
+        assert np.array_equal(out[True][k], out[False][k]), k
+        assert np.array_equal(out[True][k], out[True][0]), k
+
+
 def test_tgv_rk3_steps(tgv16):
     """Ten SSP-RK3 steps of the TGV, hybrid scheme, dt 5e-3 (config 1 at 16^3):
     primitive/property recompute, sensors, RHS and RK update all on the GPU,
