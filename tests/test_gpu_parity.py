@@ -408,6 +408,55 @@ def test_tgv_rk3_steps(tgv16):
     assert max(e) <= 1e-11
 
 
+def test_tgv32_constant_cfl_steps():
+    """Config 1 at its stated size: the 32^3 Taylor-Green vortex, ten SSP-RK3
+    steps at constant CFL 0.5 with the step formed on the device (the radius
+    of the step's start state by the allocation-free reduction into device
+    memory, then per stage the full RHS and the fused stage reading it), no
+    host read-back inside the loop; the oracle runs the reference sequence
+    (src/prometeo.rg:1159-1165, 1407-1470) with dt = CFL / radius on the host."""
+    import htr_amd as H
+    n = (32, 32, 32)
+    st, meta = tgv_state(32)
+    st, om = complete_state(st, meta, n)
+    say("case tgv32 10 ConstantCFL RK3 steps, device dt")
+    cfl = 0.5
+    g, f = oracle_fields(st, 1, n)
+    fs = to_gpu(st, 1, n)
+    gm = gpu_mix(meta)
+    radius = torch.zeros(1, dtype=torch.float64, device="cuda")
+    RKC = [(1.0, 0.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0)]
+    dts = []
+    for step in range(10):
+        O.ducros_sensor(g, f, 1.0)
+        for d in range(3):
+            O.shock_sensor(g, f, d, om)
+        dt = cfl / O.max_spectral_radius(g, f, om)
+        dts.append(dt)
+        f["Conserved_old"] = f["Conserved"]
+        for s in range(3):
+            c = RKC[s]
+            O.full_rhs(g, f, O.SCHEME_HYBRID, c[0], c[1], dt * c[2], 0, om)
+            O.rk_update(g, f, 5, s, dt)
+            O.update_primitive(g, f, om)
+            O.update_properties(g, f, om)
+        H.update_ducros_sensor(fs, 1.0)
+        for d in range(3):
+            H.update_shock_sensor(fs, d, gm)
+        H.max_spectral_radius_noalloc(fs, gm, radius)
+        H.rk_init(fs, 5)
+        for s in range(3):
+            c = RKC[s]
+            H.full_rhs_cfl(fs, 0, (c[0], c[1]), cfl, c[2], radius, False, gm)
+            H.rk_stage(fs, s, gm, cfl=cfl, radius=radius)
+    r_last = float(radius.item())
+    e = norm_err(host(fs["Conserved"]), f["Conserved"])
+    say("  dt %.4e .. %.4e; last radius gpu %.15e oracle-dt %.15e; after 10 steps: %s"
+        % (dts[0], dts[-1], r_last, cfl / dts[-1], ["%.2e" % x for x in e]))
+    assert abs(cfl / r_last - dts[-1]) <= 1e-12 * dts[-1]
+    assert max(e) <= 1e-11
+
+
 def test_tile_subrect_absolute_coordinates(air16, parity_record):
     """A tile-like call: execution rect [3, 12]^3 inside a 16^3 periodic box
     whose Fluid_bounds start at absolute (100, -7, 40).  Euler (hybrid, every
