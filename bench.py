@@ -85,8 +85,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the per-cell-composition and boundary-layer readings of the default run")
     ap.add_argument("--composition", default="per-seed", choices=["per-seed", "per-cell"])
-    ap.add_argument("--layout", default="soa", choices=["soa", "aos"],
-                    help="vector fields structure-of-arrays, or per point as the reference's VecNEq/VecNSp/Vec3")
+    ap.add_argument("--layout", default="soa", choices=["soa", "aos", "padded"],
+                    help="vector fields structure-of-arrays, per point as the reference's VecNEq/VecNSp/Vec3, "
+                         "or structure-of-arrays with padded (non power-of-two) strides")
     ap.add_argument("--workload", default="air", choices=["air", "bl"],
                     help="air: 256^3 periodic air box; bl: 512x256x128 Mach 6 boundary layer")
     ap.add_argument("--chem-n", type=int, default=32,

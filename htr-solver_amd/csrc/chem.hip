@@ -276,8 +276,11 @@ struct ChemGroup {
       const double k2 = vel[0]*vel[0] + vel[1]*vel[1] + vel[2]*vel[2];
       const double e = x[NS+3]*irho - 0.5*k2;
       const double Tr = T*m.TRef, rr = rho*m.rhoRef;
+      double *rw = v(Lay::ERR);                  // rhoRef/W_k
+      for (int e = lane; e < N*N; e += L) J[e] = 0.0;
       for (int i = lane; i < NS; i += L) {
          const DSpec &sp = m.sp[i];
+         rw[i] = m.rhoRef/sp.W;
          const double y = irho*x[i];
          const double yc = dmin(dmax(y, 1e-60), 1.0);
          const bool f = (yc == y);
@@ -349,12 +352,13 @@ struct ChemGroup {
          Dq[k] = tb ? D : 0.0;
       }
       gsync();
-      // rows: lane over species rows; the sparse part of row i (the
-      // concentrations of the species of its reactions) is accumulated into
-      // the row, the dense parts (dT, dctot, clamped species) added last
-      for (int i = lane; i < N; i += L) {
-         for (int j = 0; j < N; j++) J[i + j*N] = 0.0;
-         if (i >= NS) continue;
+      // rows (lanes over species rows): the row's sums of dq/dT and dq/dctot
+      // over its reactions and the clamped-species term into LDS, the sparse
+      // part (the concentrations of the species of its reactions) into the
+      // zeroed row; then every entry with the dense parts (lanes over the
+      // N x N entries)
+      double *aTv = v(Lay::G2), *aCv = v(Lay::G3), *sclv = v(Lay::G4);
+      for (int i = lane; i < NS; i += L) {
          double aT = 0.0, aC = 0.0, scl = 0.0;
          for (int e2 = m.spr_ptr[i]; e2 < m.spr_ptr[i+1]; e2++) {
             const int r = m.spr_reac[e2];
@@ -367,7 +371,7 @@ struct ChemGroup {
                for (int l = 0; l < rc.Ne; l++)
                   if (l != q) p *= (rc.ed_nu[l] == 1.0f) ? C[rc.ed_ind[l]] : pow(C[rc.ed_ind[l]], (double)rc.ed_nu[l]);
                const int sp = rc.ed_ind[q];
-               const double val = nu*CKF[r]*p*m.rhoRef/m.sp[sp].W;
+               const double val = nu*CKF[r]*p*rw[sp];
                if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
             }
             if (rc.has_backward)
@@ -376,19 +380,25 @@ struct ChemGroup {
                   for (int l = 0; l < rc.Np; l++)
                      if (l != q) p *= (rc.pd_nu[l] == 1.0f) ? C[rc.pd_ind[l]] : pow(C[rc.pd_ind[l]], (double)rc.pd_nu[l]);
                   const int sp = rc.pd_ind[q];
-                  const double val = -nu*CKB[r]*p*m.rhoRef/m.sp[sp].W;
+                  const double val = -nu*CKB[r]*p*rw[sp];
                   if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
                }
             if (rc.kind == REAC_THIRDB && rc.Ntb > 0)
                for (int q = 0; q < rc.Ntb; q++) {
                   const int sp = rc.tb_ind[q];
-                  const double val = nu*Dq[r]*(rc.tb_eff[q] - 1.0)*m.rhoRef/m.sp[sp].W;
+                  const double val = nu*Dq[r]*(rc.tb_eff[q] - 1.0)*rw[sp];
                   if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
                }
          }
-         const double sc = m.sp[i].W*m.iwiRef;
-         for (int j = 0; j < N; j++)
-            J[i + j*N] = sc*(J[i + j*N] + aT*dT[j] + aC*dct[j] + ((j < NS) ? scl : 0.0));
+         aTv[i] = aT;
+         aCv[i] = aC;
+         sclv[i] = scl;
+      }
+      gsync();
+      for (int e = lane; e < N*N; e += L) {
+         const int i = e % N, j = e/N;
+         if (i < NS)
+            J[e] = (m.sp[i].W*m.iwiRef)*(J[e] + aTv[i]*dT[j] + aCv[i]*dct[j] + ((j < NS) ? sclv[i] : 0.0));
       }
       gsync();
       jac_valid = true;
@@ -460,6 +470,73 @@ struct ChemGroup {
       }
    }
 
+   // ludcmp with one row per lane (L == 64 >= N): lane i holds row i of the
+   // matrix in registers and keeps it; its current position (the row index
+   // after the interchanges so far) is tracked instead of moving the data.
+   // Per column the candidates are the rows at positions >= j, the pivot the
+   // last position attaining the maximum scaled magnitude (the reference's
+   // `>=' scan), the pivot row is broadcast by readlane, and every entry
+   // receives the same products in the same order as ludcmp above, so the
+   // factors and the pivot indices are the same.  The factors are written to
+   // Mout in the reference's layout (row = final position).  The row of lane
+   // i is a(k) for k < N.
+   template<typename T_, typename F_>
+   __device__ __forceinline__ void ludcmp_reg(T_ *Mout, F_ a)
+   {
+      static_assert(L == 64 && N <= 64, "one row per lane");
+      int *piv = ind();
+      const bool act = lane < N;
+      T_ row[N];
+#pragma unroll
+      for (int k = 0; k < N; k++) row[k] = act ? a(k) : (T_)0;
+      T_ big = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) { const T_ x = fabs(row[k]); big = (big < x) ? x : big; }
+      const T_ vvi = (T_)1/big;
+      int pos = lane;
+#pragma unroll
+      for (int j = 0; j < N; j++) {
+         // pivot: the last position >= j with the largest vv |M(i, j)|
+         const T_ dum = vvi*fabs(row[j]);
+         const bool cand = act && pos >= j && ((double)dum >= -1.0);
+         double best = cand ? (double)dum : -1.0;
+         int bp = cand ? pos : -1;
+#pragma unroll
+         for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, 64);
+            const int op = __shfl_xor(bp, o, 64);
+            if (ob > best || (ob == best && op > bp)) { best = ob; bp = op; }
+         }
+         const int imax = __builtin_amdgcn_readfirstlane((bp < 0) ? j : bp);
+         const unsigned long long mw = __ballot(act && pos == imax);
+         const unsigned long long mj = __ballot(act && pos == j);
+         const int P = __ffsll((long long)mw) - 1, Pj = __ffsll((long long)mj) - 1;
+         if (lane == P) pos = j;
+         else if (lane == Pj) pos = imax;
+         if (lane == 0) piv[j] = imax;
+         T_ pv = (sizeof(T_) == 8) ? (T_)rdlane((double)row[j], P) : (T_)rdlanef((float)row[j], P);
+         if (pv == (T_)0) {
+            pv = (T_)1.0e-20;
+            if (lane == P) row[j] = pv;
+         }
+         if (j != N - 1) {
+            const T_ dv = (T_)1/pv;
+            const bool below = act && pos > j;
+            if (below) row[j] *= dv;
+#pragma unroll
+            for (int k = j + 1; k < N; k++) {
+               const T_ ujk = (sizeof(T_) == 8) ? (T_)rdlane((double)row[k], P) : (T_)rdlanef((float)row[k], P);
+               if (below) row[k] -= row[j]*ujk;
+            }
+         }
+      }
+      if (act) {
+#pragma unroll
+         for (int k = 0; k < N; k++) Mout[pos + k*N] = row[k];
+      }
+      gsync();
+   }
+
    // LUdec::lubksb (math_utils.hpp:157-188): permutation, forward
    // substitution column by column (each entry receives its products in the
    // reference's order), back substitution row by row by one lane in the
@@ -496,11 +573,16 @@ struct ChemGroup {
       if (PREC == HTR_CHEM_FP64) {
          double *M = reinterpret_cast<double*>(A);
          const double *J = Jm();
-         for (int e = lane; e < N*N; e += L) M[e] = -J[e];
-         gsync();
-         for (int i = lane; i < N; i += L) M[i + i*N] += d;
-         gsync();
-         ludcmp<double>(M);
+         if constexpr (L == 64 && N <= 64) {
+            const int i = lane;
+            ludcmp_reg<double>(M, [&](int k) { double x = -J[i + k*N]; if (k == i) x += d; return x; });
+         } else {
+            for (int e = lane; e < N*N; e += L) M[e] = -J[e];
+            gsync();
+            for (int i = lane; i < N; i += L) M[i + i*N] += d;
+            gsync();
+            ludcmp<double>(M);
+         }
       } else {
          // equilibrate by powers of two (rows, then columns) before rounding
          // to FP32
@@ -519,12 +601,24 @@ struct ChemGroup {
          }
          gsync();
          float *M = reinterpret_cast<float*>(A);
-         for (int e = lane; e < N*N; e += L) {
-            const int i = e % N, j = e/N;
-            M[e] = (float)(rs[i]*(-J[e] + ((i == j) ? d : 0.0))*cs[j]);
+         if constexpr (L == 64 && N <= 64) {
+            const int i = lane;
+            const double rsi = (i < N) ? rs[i] : 0.0;
+            ludcmp_reg<float>(M, [&](int k) { return (float)(rsi*(-J[i + k*N] + ((i == k) ? d : 0.0))*cs[k]); });
+         } else {
+            for (int e = lane; e < N*N; e += L) {
+               const int i = e % N, j = e/N;
+               M[e] = (float)(rs[i]*(-J[e] + ((i == j) ? d : 0.0))*cs[j]);
+            }
+            gsync();
+            ludcmp<float>(M);
          }
-         gsync();
-         ludcmp<float>(M);
+         if (L == 64 && N <= 64) {
+            // reciprocal pivots of the register refinement
+            double *ud = v(Lay::VV);
+            for (int i = lane; i < N; i += L) ud[i] = (double)(1.0f/M[i + i*N]);
+            gsync();
+         }
          lu64 = false;
       }
    }
@@ -553,15 +647,21 @@ struct ChemGroup {
       return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
    }
 
-   // FP32_IR refinement with one row per lane (L == 64 >= N): the solution,
-   // the residual and the correction in registers, the FP32 factors and the
-   // FP64 Jacobian read from LDS, operands broadcast by readlane.  Returns
-   // false when the refinement stalls (b then holds the right-hand side).
+   __device__ __forceinline__ static float rdlanef(float x, int l)
+   {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+   }
+
+   // FP32_IR refinement with one row per lane (L == 64 >= N): the solution
+   // and the FP64 residual in registers, the correction solved in FP32 with
+   // the FP32 factors (read from LDS, reciprocal pivots formed once per
+   // factorisation), operands broadcast by readlane.  Returns false when the
+   // refinement stalls (b then holds the right-hand side).
    __device__ __forceinline__ bool solve_ir_reg(double *b, double dt)
    {
       const float *M = reinterpret_cast<const float*>(A);
       const double *J = Jm();
-      const double *rs = v(Lay::RS), *cs = v(Lay::CS);
+      const double *rs = v(Lay::RS), *cs = v(Lay::CS), *ud = v(Lay::VV);
       const int *piv = ind();
       const int i = lane;
       const bool act = i < N;
@@ -578,23 +678,23 @@ struct ChemGroup {
                if (act) r -= (((i == j) ? d : 0.0) - J[i + j*N])*xj;
             }
          }
-         r *= rsi;
+         float rf = (float)(r*rsi);
          for (int k = 0; k < N; k++) {          // row interchanges, in order
             const int ip = __builtin_amdgcn_readfirstlane(piv[k]);
-            const double vk = rdlane(r, k), vp = rdlane(r, ip);
-            if (i == k) r = vp;
-            if (i == ip) r = vk;
+            const float vk = rdlanef(rf, k), vp = rdlanef(rf, ip);
+            if (i == k) rf = vp;
+            if (i == ip) rf = vk;
          }
          for (int k = 0; k < N - 1; k++) {      // unit lower triangle
-            const double bk = rdlane(r, k);
-            if (act && i > k) r -= (double)M[i + k*N]*bk;
+            const float bk = rdlanef(rf, k);
+            if (act && i > k) rf -= M[i + k*N]*bk;
          }
          for (int k = N - 1; k >= 0; k--) {     // upper triangle
-            const double xkk = rdlane(r, k)/(double)M[k + k*N];
-            if (i == k) r = xkk;
-            else if (act && i < k) r -= (double)M[i + k*N]*xkk;
+            const float xkk = rdlanef(rf, k)*(float)ud[k];
+            if (i == k) rf = xkk;
+            else if (act && i < k) rf -= M[i + k*N]*xkk;
          }
-         const double ci = act ? csi*r : 0.0;
+         const double ci = act ? csi*(double)rf : 0.0;
          xk += ci;
          const double nc = gmax(fabs(ci)), nx = gmax(fabs(xk));
          if (it == 0) { prev = nc; continue; }

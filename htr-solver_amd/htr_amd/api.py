@@ -138,30 +138,51 @@ class FieldSet:
         self.bounds_lo = tuple(bounds_lo)
         kw = dict(device=device)
         self.t = {}
+        if layout == "padded":
+            # structure of arrays with padded strides (one element-stride
+            # triple for every field): rows of nx + 16 elements (128-byte
+            # aligned in FP64), planes of ny + 1 rows, components one line
+            # apart, and each field's base staggered by a few cache lines, so
+            # that no stride is a power of two and the same cell of different
+            # rows, planes and fields falls in different cache sets
+            sy = nx + 16
+            sz = sy * (ny + 1)
+            sc = sz * nz + 16
+            nfield = [0]
+
+            def alloc(nc, dtype, fill):
+                stag = (nfield[0] % 8) * 40 * 8 // torch.empty((), dtype=dtype).element_size()
+                nfield[0] += 1
+                base = torch.full((max(nc, 1) * sc + stag,), fill, dtype=dtype, **kw)
+                if nc == 0:
+                    return base.as_strided((nz, ny, nx), (sz, sy, 1), stag)
+                return base.as_strided((nc, nz, ny, nx), (sc, sz, sy, 1), stag)
+        else:
+            def alloc(nc, dtype, fill):
+                shape = (nz, ny, nx) if nc == 0 else (nc, nz, ny, nx)
+                if nc and layout == "aos":
+                    return torch.full((nz, ny, nx, nc), fill, dtype=dtype, **kw).permute(3, 0, 1, 2)
+                return torch.full(shape, fill, dtype=dtype, **kw)
         for k in SCALARS:
-            self.t[k] = torch.zeros((nz, ny, nx), dtype=torch.float64, **kw)
+            self.t[k] = alloc(0, torch.float64, 0)
         for k, c in VECTORS.items():
             nc = {"ns": nspec, "neq": nspec + 4}.get(c, c)
-            if layout == "aos":
-                self.t[k] = torch.zeros((nz, ny, nx, nc), dtype=torch.float64, **kw).permute(3, 0, 1, 2)
-            else:
-                self.t[k] = torch.zeros((nc, nz, ny, nx), dtype=torch.float64, **kw)
+            self.t[k] = alloc(nc, torch.float64, 0)
         for d in range(3):
-            self.t["nType%d" % d] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+            self.t["nType%d" % d] = alloc(0, torch.int32, 0)
             for k in ("m_e", "m_d", "m_s"):
-                self.t["%s%d" % (k, d)] = torch.ones((nz, ny, nx), dtype=torch.float64, **kw)
-            self.t["shockSensor%d" % d] = torch.ones((nz, ny, nx), dtype=torch.uint8, **kw)
+                self.t["%s%d" % (k, d)] = alloc(0, torch.float64, 1)
+            self.t["shockSensor%d" % d] = alloc(0, torch.uint8, 1)
             if decisions:
-                self.t["decisions%d" % d] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+                self.t["decisions%d" % d] = alloc(0, torch.int32, 0)
         if nsteps:
-            self.t["nsteps"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+            self.t["nsteps"] = alloc(0, torch.int32, 0)
         if flags:
-            self.t["flags"] = torch.zeros((nz, ny, nx), dtype=torch.int32, **kw)
+            self.t["flags"] = alloc(0, torch.int32, 0)
         if profiles:
             for k, c in PROFILES.items():
                 nc = {"ns": nspec}.get(c, c)
-                shape = (nz, ny, nx) if nc == 0 else (nc, nz, ny, nx)
-                self.t[k] = torch.zeros(shape, dtype=torch.float64, **kw)
+                self.t[k] = alloc(nc, torch.float64, 0)
         self._c = None
 
     def __getitem__(self, k):
