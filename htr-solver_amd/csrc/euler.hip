@@ -1026,7 +1026,12 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
       }
    };
 
-   // component waves: components wv, wv + 3, wv + 6 of chunk c into buffer bb
+   // component waves: the components of chunk c that the wave owns into
+   // buffer bb.  Owner of component i: the pattern 1, 2, 0, 1, 2 repeated
+   // (nine components: wave 0 two, wave 1 four, wave 2 three), since wave 0
+   // also back-projects the previous chunk (timed per role, an even three
+   // each made wave 0 the critical path of every chunk).
+   auto owner = [](int i) { const int r = i % 5; return (r == 2) ? 0 : (r == 0 || r == 3) ? 1 : 2; };
    auto components = [&](uint32_t c, int bb) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
@@ -1049,7 +1054,8 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
       double wj[NS];
 #pragma unroll
       for (int j = 0; j < NS; j++) wj[j] = (J.Coeff - b.rec[(int64_t)(NS + 8 + j)*cap + w])*J.iaRoe2;
-      for (int i = wv; i < NEQ; i += LOADER) {
+      for (int i = 0; i < NEQ; i++) {
+         if (owner(i) != wv) continue;             // uniform
          // the point values are re-read from LDS and the species sums
          // re-formed for every component: held in registers across the
          // loop they would cost some 120 VGPRs
