@@ -912,6 +912,8 @@ struct TenoStage {
    double  pt[6*NPV][64];
    double  fc[2][NEQ][64];
    uint8_t bits[2][NEQ][64];
+   uint32_t fid[2][64];                             // face id and node type of the chunk's faces,
+   int32_t  ty[2][64];                              // by chunk parity (left by the loader)
 };
 
 template<class MX, int DIR, int SCHEME, bool LISTED>
@@ -942,20 +944,28 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    const uint32_t K = (L < nchunks) ? (nchunks - 1u - L)/gridDim.x + 1u : 0u;
    auto chunk = [&](uint32_t k) { return L + k*gridDim.x; };
 
-   auto face_of = [&](uint32_t w, P3 &q, int &t) {
-      q = Faces<DIR>::cell(g, f, LISTED ? (uint32_t)list[w] : w);
-      t = ld<int>(f.nType[DIR], q);
+   // the face's cell and node type, from the ids and node types the loader
+   // left in LDS (buffer bb): no dependent global loads before the face's work
+   auto face_lds = [&](int bb, P3 &q, int &t) {
+      q = Faces<DIR>::cell(g, f, S.fid[bb][lane]);
+      t = S.ty[bb][lane];
    };
 
-   // loader: the stencil-point values of the 64 faces of chunk c into pv
-   auto gather = [&](uint32_t c, double *pv) {
+   // loader: the stencil-point values of the 64 faces of chunk c into pv,
+   // the face id and node type into fi, ti
+   auto gather = [&](uint32_t c, double *pv, uint32_t &fi, int &ti) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
       P3 P[6];
+      fi = 0;
+      ti = 0;
       if (valid) {
          P3 q;
          int t;
-         face_of(w, q, t);
+         fi = LISTED ? (uint32_t)list[w] : w;
+         q = Faces<DIR>::cell(g, f, fi);
+         t = ld<int>(f.nType[DIR], q);
+         ti = t;
          E.points(P, q, t);
       }
       if (!valid)
@@ -973,9 +983,11 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
                           : (v == NS + 4) ? ld<double>(f.velocity, P[k], iN) : ld<double>(f.pressure, P[k]);
       }
    };
-   auto hand_over = [&](const double *pv) {
+   auto hand_over = [&](const double *pv, uint32_t fi, int ti, int bb) {
 #pragma unroll
       for (int k = 0; k < 6*NPV; k++) S.pt[k][lane] = pv[k];
+      S.fid[bb][lane] = fi;
+      S.ty[bb][lane] = ti;
    };
 
    // loader: back-projection, limiter test, face flux and decision word of
@@ -987,7 +999,7 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
       if (w < n) {
          P3 q;
          int t;
-         face_of(w, q, t);
+         face_lds(bb, q, t);
          typename E_t::Roe R;
          E_t::get_rec(b.rec, cap, w, R);
          const double dedp = 1.0/R.dpde;
@@ -1012,7 +1024,7 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
                   limit = true;
             }
          }
-         const uint32_t fid = LISTED ? (uint32_t)list[w] : w;
+         const uint32_t fid = S.fid[bb][lane];
          store_face<NEQ>(F, nfaces, fid, Flux);
          if (dec.ptr) at<uint32_t>(dec, q) = h << 2;
       }
@@ -1036,7 +1048,7 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
       int tt = 0;
-      if (valid) { P3 q; face_of(w, q, tt); }
+      if (valid) tt = S.ty[bb][lane];
       const bool all_std = __ballot(tt != Std_node) == 0ull;
       if (!valid) return;
       typename E_t::Roe R;
@@ -1112,15 +1124,19 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    if (wv == LOADER) {
       {
          double pv[6*NPV];
-         gather(chunk(0), pv);
-         hand_over(pv);
+         uint32_t fi;
+         int ti;
+         gather(chunk(0), pv, fi, ti);
+         hand_over(pv, fi, ti, 0);
       }
       __syncthreads();
       for (uint32_t k = 0; k < K; k++) {
          double pv[6*NPV];                      // live from the gather to the hand-over only
-         if (k + 1 < K) gather(chunk(k + 1), pv);
+         uint32_t fi = 0;
+         int ti = 0;
+         if (k + 1 < K) gather(chunk(k + 1), pv, fi, ti);
          __syncthreads();                       // chunk k's components are in LDS, its point values are free
-         if (k + 1 < K) hand_over(pv);
+         if (k + 1 < K) hand_over(pv, fi, ti, (int)((k + 1u) & 1u));
          __syncthreads();                       // chunk k + 1's point values are in LDS
       }
    } else {

@@ -227,6 +227,8 @@ inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x);
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline long long __double_as_longlong(double x) { return ::hipemu::unbits<long long>(::hipemu::bits(x)); }
 inline double __longlong_as_double(long long x) { return ::hipemu::unbits<double>(::hipemu::bits(x)); }
+inline int __float_as_int(float x) { int i; memcpy(&i, &x, sizeof i); return i; }
+inline float __int_as_float(int i) { float x; memcpy(&x, &i, sizeof x); return x; }
 template<class T> inline T atomicAdd(T *p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline double atomicAdd(double *p, double v)
 {
