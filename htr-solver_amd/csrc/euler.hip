@@ -1039,11 +1039,11 @@ euler_teno_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g,
    };
 
    // component waves: the components of chunk c that the wave owns into
-   // buffer bb.  Owner of component i: the pattern 1, 2, 0, 1, 2 repeated
-   // (nine components: wave 0 two, wave 1 four, wave 2 three), since wave 0
+   // buffer bb.  Owner of component i: the pattern 1, 2, 1, 2, 0, 2, 1, 2, 1
+   // (nine components: wave 0 one, waves 1 and 2 four each), since wave 0
    // also back-projects the previous chunk (timed per role, an even three
    // each made wave 0 the critical path of every chunk).
-   auto owner = [](int i) { const int r = i % 5; return (r == 2) ? 0 : (r == 0 || r == 3) ? 1 : 2; };
+   auto owner = [](int i) { const int r = i % 9; return (r == 4) ? 0 : (r & 1) ? 2 : 1; };
    auto components = [&](uint32_t c, int bb) {
       const uint32_t w = c*64u + (uint32_t)lane;
       const bool valid = w < n;
