@@ -881,14 +881,15 @@ euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, 
 
 // Passes 2b + 2c in one kernel, pipelined: a block of four waves takes
 // chunks of 64 listed TENO faces.
-//   components  waves 0, 1, 2: wave w forms components w, w + 3, w + 6 of the
-//               64 faces of chunk k from the stencil-point values in LDS (the
-//               component index is wave-uniform, so every branch on it is
-//               uniform): per point the species sums of the left-eigenvector
-//               products (rhs.inl:443-451), the split fluxes on row i, TENO+
-//               and TENO-, -(F+ + F-)/2 and the stencil-selection bits into
-//               LDS (double-buffered by chunk parity);
-//   finish      wave 0, before its components of chunk k: back-projects
+//   components  waves 0, 1, 2 form the components they own (wave 0 one of
+//               nine, waves 1 and 2 four each) of the 64 faces of chunk k
+//               from the stencil-point values in LDS (the component index is
+//               wave-uniform, so every branch on it is uniform): per point
+//               the species sums of the left-eigenvector products
+//               (rhs.inl:443-451), the split fluxes on row i, TENO+ and
+//               TENO-, -(F+ + F-)/2 and the stencil-selection bits into LDS
+//               (double-buffered by chunk parity);
+//   finish      wave 0, before its component of chunk k: back-projects
 //               chunk k - 1 (add_column, the reference's summation order),
 //               tests the limiter, writes the face fluxes and decision words
 //               and lists the faces that need the limiter blend;
@@ -896,7 +897,8 @@ euler_roe_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, 
 //               stencil-point values (conserved vector, u_n, p: 6 (nEq + 2)
 //               doubles per face) into its registers;
 //   hand-over   after the component waves are done with chunk k, the loader
-//               writes chunk k + 1's values into LDS.
+//               writes chunk k + 1's values, face ids and node types into
+//               LDS.
 // The scattered gathers thus fly while the components of the previous chunk
 // are computed; only the LDS write of the hand-over is on the component
 // waves' path.  The Roe record of a face (euler_roe_kernel) is read by the
