@@ -543,6 +543,13 @@ def main():
         result["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                   "sample": "%d^3 box of the same workload, %d full RHS in %.1f s"
                                             % (a.cpu_sample, reps, el)}
+        # the same restatement on one core (a smaller box, same workload)
+        from oracle import oracle as O
+        O.set_threads(1)
+        rate1, reps1, el1 = cpu_baseline(24, a.seed, scheme, a.cpu_seconds / 2, a.composition)
+        O.set_threads(threads)
+        result["cpu_baseline"]["one_core"] = {"value": rate1, "cores": 1,
+                                              "sample": "24^3 box, %d full RHS in %.1f s" % (reps1, el1)}
     print(json.dumps(result), flush=True)
 
 

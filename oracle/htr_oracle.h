@@ -267,6 +267,9 @@ void   o_chem_jacobian(const o_mix *m, const double *x, const double *Conserved_
                        double *temperature, int jacobian, double *J);
 int    o_chem_jacobian_analytic(const o_mix *m, const double *x, double Tn, double *J);
 
+/* threads of the OpenMP loops (n > 0); returns the count in force */
+int    o_set_threads(int n);
+
 /* ---- test hooks: internal steps of the flux reconstruction -------------- */
 /* Roe state (ComputeRoeAverages, src/prometeo_rhs.inl:200-279) of the left and
  * right states L, R = [rho, u, v, w, p, rhoE, Y_0 .. Y_ns-1];

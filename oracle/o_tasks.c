@@ -22,6 +22,9 @@
 #include <string.h>
 #include <stdlib.h>
 #include "htr_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static inline double dmax(double a, double b) { return (a < b) ? b : a; }
 static inline double dmin(double a, double b) { return (b < a) ? b : a; }
@@ -1035,4 +1038,16 @@ double o_max_mach(const o_grid *g, const o_fields *f, int dir)
             acc = dmax(acc, fabs(f->velocity[dir*N + p])/f->SoS[p]);
          }
    return acc;
+}
+
+/* threads of the OpenMP loops (the CPU baseline's one-core reading) */
+int o_set_threads(int n)
+{
+#ifdef _OPENMP
+   if (n > 0) omp_set_num_threads(n);
+   return omp_get_max_threads();
+#else
+   (void)n;
+   return 1;
+#endif
 }

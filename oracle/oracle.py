@@ -131,6 +131,7 @@ def _declare(L):
         "o_rosenbrock_testprob": (None, [_dp, d, d, C.POINTER(C.c_int32)]),
         "o_chem_cell": (i, [vp, _dp, _dp, _dp, d]),
         "o_chem_cell_ex": (i, [vp, _dp, _dp, _dp, d, i, d]),
+        "o_set_threads": (i, [i]),
         "o_chem_update_ex": (None, [C.POINTER(OGrid), C.POINTER(OFields), d, vp, C.POINTER(C.c_int32), i, d]),
         "o_chem_jacobian": (None, [vp, _dp, _dp, _dp, i, _dp]),
         "o_test_roe": (None, [vp, _dp, _dp, _dp]),
@@ -388,6 +389,11 @@ def chem_source(g, f, m):
 
 
 JAC_FD, JAC_ANALYTIC = 0, 1
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (n > 0); returns the count in force."""
+    return lib().o_set_threads(int(n))
 
 
 def chem_update(g, f, dt, m, nsteps=None, jacobian=JAC_FD, tol=1e-10):
