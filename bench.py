@@ -95,7 +95,9 @@ def parse():
     ap.add_argument("--chem-dt", type=float, default=1e-5)
     ap.add_argument("--chem-only", action="store_true",
                     help="only the batched-chemistry measurement (its own JSON line; --chem-n, --chem-dt)")
-    ap.add_argument("--chem-tune", action="store_true", help="run the launch-shape autotuner first")
+    ap.add_argument("--no-chem-tune", dest="chem_tune", action="store_false",
+                    help="skip the launch-shape autotuner of the chemistry (default: each path's shape is chosen "
+                         "by timing the candidates on a quarter of the cells)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--overlap", default="off", choices=["on", "off"],
