@@ -185,7 +185,7 @@ struct ChemGroup {
       const double logRP = log(RGAS/Pr);
       for (int k = lane; k < m.nReac; k += L) {
          const DReac &rc = m.rc[k];
-         const double Kf = MX::RateCoeff(rc.A, rc.n, rc.EovR, Tr);
+         const double Kf = MX::RateCoeff(rc.A, rc.n, rc.EovR, Tr, logT);
          double a = 1.0;
          for (int i = 0; i < rc.Ne; i++) {
             const int id = rc.ed_ind[i];
@@ -312,9 +312,10 @@ struct ChemGroup {
       }
       // per reaction: dq/dT (per nondimensional T), dq/dctot, c Kf, c Kb and
       // Kf a - Kb b (third-body term)
+      const double logT = log(Tr);
       for (int k = lane; k < nR; k += L) {
          const DReac &rc = m.rc[k];
-         const double Kf = MX::RateCoeff(rc.A, rc.n, rc.EovR, Tr);
+         const double Kf = MX::RateCoeff(rc.A, rc.n, rc.EovR, Tr, logT);
          double dlKf = 0.0;
          if (rc.n != 0.0f) dlKf += (double)rc.n/Tr;
          if (fabs((double)rc.EovR) > 1e-5) dlKf += (double)rc.EovR/(Tr*Tr);

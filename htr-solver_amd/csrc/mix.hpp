@@ -369,13 +369,15 @@ struct Mix {
       for (int i = 0; i < NS; i++) d[i] = (RGAS*T/m.sp[i].W + (gamma - 1)*(e - d[i]))*m.ieRef;
    }
 
-   // Reaction.inl:34-42
-   HD static double RateCoeff(double A, float n, float EovR, double T)
+   // Reaction.inl:34-42: Kf = A T^n exp(-EovR/T), formed as
+   // A exp(n ln T - EovR/T) from logT = ln T: one exponential in place of
+   // pow and exp, the same function up to rounding
+   HD static double RateCoeff(double A, float n, float EovR, double T, double logT)
    {
-      double Kf = A;
-      if (n != 0.0) Kf *= pow(T, (double)n);
-      if (fabs((double)EovR) > 1e-5) Kf *= exp((double)(-EovR)/T);
-      return Kf;
+      double arg = 0.0;
+      if (n != 0.0) arg = (double)n*logT;
+      if (fabs((double)EovR) > 1e-5) arg -= (double)EovR/T;
+      return A*exp(arg);
    }
    // MultiComponent.inl:371-417 with Reaction.inl:82-228 (no fall-off)
    HD static void ProductionRates(const DMix &m, double *w, double rhon, double Pn, double Tn, const double *Y)
@@ -394,7 +396,7 @@ struct Mix {
       const double logRP = log(RGAS/P);
       for (int k = 0; k < m.nReac; k++) {
          const DReac &r = m.rc[k];
-         const double Kf = RateCoeff(r.A, r.n, r.EovR, T);
+         const double Kf = RateCoeff(r.A, r.n, r.EovR, T, logT);
          double a = 1.0;
          for (int i = 0; i < r.Ne; i++) {
             const int ind = r.ed_ind[i];

@@ -389,13 +389,15 @@ void o_Getdpdrhoi(const o_mix *m, double *dpdrhoi, double gamma, double Tn, cons
 /* ------------------------------------------------------------------------- */
 /* Kinetics (src/Mixtures/Reaction.inl)                                       */
 /* ------------------------------------------------------------------------- */
-/* Reaction.inl:34-42 */
+/* Reaction.inl:34-42: Kf = A T^n exp(-EovR/T), formed as A exp(n ln T - EovR/T):
+   the same function with one exponential in place of pow and exp (the two
+   forms differ by rounding, a few 1e-15 relative) */
 static double rate_coeff(const o_arrhenius *a, double T)
 {
-   double Kf = a->A;
-   if (a->n != 0.0) Kf *= pow(T, (double)a->n);
-   if (fabs((double)a->EovR) > 1e-5) Kf *= exp((double)(-a->EovR)/T);
-   return Kf;
+   double arg = 0.0;
+   if (a->n != 0.0) arg = (double)a->n*log(T);
+   if (fabs((double)a->EovR) > 1e-5) arg -= (double)a->EovR/T;
+   return a->A*exp(arg);
 }
 
 /* Reaction.inl:82-99 */
