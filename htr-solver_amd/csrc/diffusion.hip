@@ -294,29 +294,38 @@ diffusion_walk_kernel(const htr_fields f, const DMix *__restrict__ mix, const Ge
    }
 }
 
-// x sweep: lanes on consecutive faces of one row.  A wave takes XD_OUT = 62
-// faces (lanes 2..63): every lane forms the transverse gradients of its
-// right cell q, and takes those of its left cell and its FluxM from lane l-1
-// (wave shuffles), so lanes 0 and 1 serve as the halo.  Face 0 takes its
-// left gradients at pm1 itself (rhs.inl:1253-1254).
+// x sweep: lanes on consecutive faces.  The faces of all rows are numbered
+// row-major (face j of row r is r (esz + 1) + j, its face id), and a wave
+// takes XD_OUT = 62 consecutive ones (lanes 2..63), so a wave may span the
+// end of one row and the start of the next: every lane forms the transverse
+// gradients of its right cell q, and takes those of its left cell and its
+// FluxM from lane l-1 (wave shuffles), so lanes 0 and 1 serve as the halo;
+// lane l-1 holds face j-1 of the same row whenever j >= 1, and face 0 takes
+// its left gradients at pm1 itself (rhs.inl:1253-1254) and adds no
+// divergence.
 constexpr int XD_OUT = 62;
 
 template<class MX, int G>
 __global__ void __launch_bounds__(256)
 diffusion_x_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g, double *__restrict__ H,
-                   uint32_t nchunk, uint32_t nwaves)
+                   uint32_t nfx, uint32_t nwaves)
 {
    using D_t = Diffusion<MX, 0>;
    using K = DiffGroup<MX, G>;
    const uint32_t wave = (blockIdx.x*blockDim.x + threadIdx.x) >> 6;
    if (wave >= nwaves) return;                                      // whole waves
    const int lane = threadIdx.x & 63;
-   const uint32_t row = wave/nchunk, k = wave - row*nchunk;
    const int esz = g.esz[0];
-   const int j = (int)k*XD_OUT + lane - 2;                           // face of this lane
+   const int gf = (int)(wave*XD_OUT) + lane - 2;                    // face id of this lane
+   const bool live = gf >= 0 && (uint32_t)gf < nfx;
+   uint32_t row = 0;
+   int j = esz + 1;                                                  // (no face)
+   if (live) {
+      row = (uint32_t)gf/(uint32_t)(esz + 1);
+      j = gf - (int)(row*(uint32_t)(esz + 1));
+   }
    const P3 prow = mkp(g, g.elo[0], g.elo[1] + (int)(row % (uint32_t)g.esz[1]),
                        g.elo[2] + (int)(row/(uint32_t)g.esz[1]));
-   const bool live = j <= esz;
    P3 p = wplus(g, 0, prow, j - 1);                                  // left cell of face j
    if (j == 0) p = wminus(g, 0, prow, c_st.Cp[ld<int>(f.nType[0], prow)][1]);
    const P3 q = wplus(g, 0, p, 1);
@@ -342,7 +351,7 @@ diffusion_x_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
             D.template deriv3<D_t::D2>(p, g2[0]);
          }
       }
-      diff_face<MX, 0, G>(D, Fc, p, q, g1, g2, H, row*(uint32_t)(esz + 1) + (uint32_t)j, lane >= 2);
+      diff_face<MX, 0, G>(D, Fc, p, q, g1, g2, H, (uint32_t)gf, lane >= 2);
    }
    double Fm[K::NC];
 #pragma unroll
@@ -367,14 +376,14 @@ static int launch_diff(const htr_fields &f, const DMix *mix, const Geo &g, hipSt
    if (rc) return rc;
    double *H = wp->faces;                          // heat flux per face
    if constexpr (DIR == 0) {
-      const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XD_OUT - 1)/XD_OUT;
-      const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
-      if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many rows");
+      // a lane's face id, below nwaves*XD_OUT + 62 <= nfaces + 123, stays an int
+      if (nfaces + 128 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "diffusion: too many faces");
+      const int64_t nwaves = (nfaces + XD_OUT - 1)/XD_OUT;
       HTR_LAUNCH("diffusion_species_x", (diffusion_x_kernel<MX, 0>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0, s,
-                 f, mix, g, H, (uint32_t)nchunk, (uint32_t)nwaves);
+                 f, mix, g, H, (uint32_t)nfaces, (uint32_t)nwaves);
       if ((rc = check_launch("diffusion_x_kernel"))) return rc;
       HTR_LAUNCH("diffusion_stress_x", (diffusion_x_kernel<MX, 1>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0, s,
-                 f, mix, g, H, (uint32_t)nchunk, (uint32_t)nwaves);
+                 f, mix, g, H, (uint32_t)nfaces, (uint32_t)nwaves);
       return check_launch("diffusion_x_kernel");
    } else {
       const int L = walk_segment(DIR, (int64_t)g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B], (int64_t)g.esz[DIR] + 1);

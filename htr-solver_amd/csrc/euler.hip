@@ -726,29 +726,38 @@ euler_kg_walk_kernel(const htr_fields f, const Geo g, const htr_view hv, const d
    kg_walk<MX, DIR, ALL_KG, G>(f, g, hv, F, nfaces, line, j0, j1, dec);
 }
 
-// x sweep: lanes on consecutive faces of one row.  A wave takes XK_OUT = 61
-// faces (lanes 3..63) and forms the new pairs S(p,p+d) of its 64 cells; a
-// lane's older pairs are those of lanes l-1 and l-2 and its FluxM is the flux
-// of lane l-1 (wave shuffles), so lanes 0..2 serve as the halo of lane 3.
+// x sweep: lanes on consecutive faces.  Each row has esz + 4 slots, faces
+// j = -3 .. esz (the three before face 0 only form pairs), numbered
+// row-major across rows, and a wave takes XK_OUT = 61 consecutive slots
+// (lanes 3..63), so a wave may span the end of one row and the start of the
+// next.  Every lane forms the new pairs S(p,p+d) of its cell; a lane's older
+// pairs are those of lanes l-1 and l-2 and its FluxM is the flux of lane
+// l-1 (wave shuffles), so lanes 0..2 serve as the halo of lane 3; for a face
+// j >= 0 those lanes hold slots j-1 and j-2 of the same row.
 constexpr int XK_OUT = 61;
 
 template<class MX, bool ALL_KG, int G>
 __global__ void __launch_bounds__(256)
 euler_kg_x_kernel(const htr_fields f, const Geo g, const htr_view hv, const double *__restrict__ F,
-                  int64_t nfaces, uint32_t nchunk, uint32_t nwaves, htr_view dec)
+                  uint32_t nslot, uint32_t nwaves, htr_view dec)
 {
    using K = KGGroup<MX, G>;
    using Cell = KGCell<MX, 0, G>;
    const uint32_t wave = (blockIdx.x*blockDim.x + threadIdx.x) >> 6;
    if (wave >= nwaves) return;                                      // whole waves
    const int lane = threadIdx.x & 63;
-   const uint32_t row = wave/nchunk, k = wave - row*nchunk;
    const int esz = g.esz[0];
-   const int j = (int)k*XK_OUT + lane - 3;                           // face of this lane
+   const int gs = (int)(wave*XK_OUT) + lane - 3;                    // slot of this lane
+   const bool live = gs >= 0 && (uint32_t)gs < nslot;
+   uint32_t row = 0;
+   int j = esz + 1;                                                  // (no face)
+   if (live) {
+      row = (uint32_t)gs/(uint32_t)(esz + 4);
+      j = gs - (int)(row*(uint32_t)(esz + 4)) - 3;                    // face of this lane
+   }
    const P3 prow = mkp(g, g.elo[0], g.elo[1] + (int)(row % (uint32_t)g.esz[1]),
                        g.elo[2] + (int)(row/(uint32_t)g.esz[1]));
    const P3 p = wplus(g, 0, prow, j - 1);                            // left cell of face j
-   const bool live = j <= esz;
    double N1[K::NP], N2[K::NP], N3[K::NP];
    int t = 0;
    if (live) {
@@ -1258,14 +1267,16 @@ static int kg_walks(const htr_fields &f, const Geo &g, const htr_view &hv, const
 {
    int rc;
    if constexpr (DIR == 0) {
-      const int64_t nchunk = ((int64_t)g.esz[0] + 1 + XK_OUT - 1)/XK_OUT;
-      const int64_t nwaves = nchunk*g.esz[1]*g.esz[2];
-      if (nwaves*64 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many rows");
+      // esz + 4 slots per row; a lane's slot, below nwaves*XK_OUT + 61 <=
+      // nslot + 121, stays an int
+      const int64_t nslot = ((int64_t)g.esz[0] + 4)*g.esz[1]*g.esz[2];
+      if (nslot + 128 >= (1ll << 31)) return set_error(HTR_E_UNSUPPORTED, "euler: too many rows");
+      const int64_t nwaves = (nslot + XK_OUT - 1)/XK_OUT;
       HTR_LAUNCH("euler_kg_species_x", (euler_kg_x_kernel<MX, ALL, 0>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
-                 s, f, g, hv, F, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
+                 s, f, g, hv, F, (uint32_t)nslot, (uint32_t)nwaves, dec);
       if ((rc = check_launch("euler_kg_x_kernel"))) return rc;
       HTR_LAUNCH("euler_kg_flow_x", (euler_kg_x_kernel<MX, ALL, 1>), dim3(grid_for(nwaves*64, 256)), dim3(256), 0,
-                 s, f, g, hv, F, nfaces, (uint32_t)nchunk, (uint32_t)nwaves, dec);
+                 s, f, g, hv, F, (uint32_t)nslot, (uint32_t)nwaves, dec);
       return check_launch("euler_kg_x_kernel");
    } else {
       const int L = walk_segment(DIR, (int64_t)g.esz[Faces<DIR>::A]*g.esz[Faces<DIR>::B], (int64_t)g.esz[DIR] + 1);
