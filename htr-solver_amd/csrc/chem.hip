@@ -189,8 +189,7 @@ struct ChemGroup {
          double a = 1.0;
          for (int i = 0; i < rc.Ne; i++) {
             const int id = rc.ed_ind[i];
-            if (rc.ed_nu[i] == 1) a *= ta[id];
-            else                  a *= pow(ta[id], (double)rc.ed_nu[i]);
+            a *= conc_pow(ta[id], rc.ed_nu[i]);
          }
          double Kb = 0.0, b = 1.0;
          if (rc.has_backward) {
@@ -201,8 +200,7 @@ struct ChemGroup {
             Kb = Kf*exp(-lnKc);
             for (int i = 0; i < rc.Np; i++) {
                const int id = rc.pd_ind[i];
-               if (rc.pd_nu[i] == 1) b *= ta[id];
-               else                  b *= pow(ta[id], (double)rc.pd_nu[i]);
+               b *= conc_pow(ta[id], rc.pd_nu[i]);
             }
          }
          double q = Kf*a - Kb*b;
@@ -322,7 +320,7 @@ struct ChemGroup {
          double a = 1.0;
          for (int i = 0; i < rc.Ne; i++) {
             const double c0 = C[rc.ed_ind[i]];
-            a *= (rc.ed_nu[i] == 1.0f) ? c0 : pow(c0, (double)rc.ed_nu[i]);
+            a *= conc_pow(c0, rc.ed_nu[i]);
          }
          double Kb = 0.0, b = 1.0, sumNu = 0.0, dlKb = 0.0;
          if (rc.has_backward) {
@@ -334,7 +332,7 @@ struct ChemGroup {
             for (int i = 0; i < rc.Np; i++) {
                const int id = rc.pd_ind[i];
                sumNu += rc.pd_nu[i]; sumNuG += rc.pd_nu[i]*Gk[id]; sdG += rc.pd_nu[i]*dG[id];
-               b *= (rc.pd_nu[i] == 1.0f) ? C[id] : pow(C[id], (double)rc.pd_nu[i]);
+               b *= conc_pow(C[id], rc.pd_nu[i]);
             }
             Kb = Kf*exp(sumNuG - sumNu*lct);
             dlKb = dlKf + sdG;
@@ -368,18 +366,18 @@ struct ChemGroup {
             aT += nu*QT[r];
             aC += nu*QC[r];
             for (int q = 0; q < rc.Ne; q++) {
-               double p = (rc.ed_nu[q] == 1.0f) ? 1.0 : (double)rc.ed_nu[q]*pow(C[rc.ed_ind[q]], (double)rc.ed_nu[q] - 1.0);
+               double p = (rc.ed_nu[q] == 1.0f) ? 1.0 : (double)rc.ed_nu[q]*conc_pow(C[rc.ed_ind[q]], rc.ed_nu[q] - 1.0f);
                for (int l = 0; l < rc.Ne; l++)
-                  if (l != q) p *= (rc.ed_nu[l] == 1.0f) ? C[rc.ed_ind[l]] : pow(C[rc.ed_ind[l]], (double)rc.ed_nu[l]);
+                  if (l != q) p *= conc_pow(C[rc.ed_ind[l]], rc.ed_nu[l]);
                const int sp = rc.ed_ind[q];
                const double val = nu*CKF[r]*p*rw[sp];
                if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];
             }
             if (rc.has_backward)
                for (int q = 0; q < rc.Np; q++) {
-                  double p = (rc.pd_nu[q] == 1.0f) ? 1.0 : (double)rc.pd_nu[q]*pow(C[rc.pd_ind[q]], (double)rc.pd_nu[q] - 1.0);
+                  double p = (rc.pd_nu[q] == 1.0f) ? 1.0 : (double)rc.pd_nu[q]*conc_pow(C[rc.pd_ind[q]], rc.pd_nu[q] - 1.0f);
                   for (int l = 0; l < rc.Np; l++)
-                     if (l != q) p *= (rc.pd_nu[l] == 1.0f) ? C[rc.pd_ind[l]] : pow(C[rc.pd_ind[l]], (double)rc.pd_nu[l]);
+                     if (l != q) p *= conc_pow(C[rc.pd_ind[l]], rc.pd_nu[l]);
                   const int sp = rc.pd_ind[q];
                   const double val = -nu*CKB[r]*p*rw[sp];
                   if (fr[sp] != 0.0) J[i + sp*N] += val; else scl += val*Yc[sp];

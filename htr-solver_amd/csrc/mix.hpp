@@ -82,6 +82,17 @@ struct DMix {
 HD double dmax(double a, double b) { return (a < b) ? b : a; }
 HD double dmin(double a, double b) { return (b < a) ? b : a; }
 
+// C^nu of a rate of progress: Reaction.inl:112-137 multiplies C for nu = 1
+// and takes pow(C, nu) otherwise; nu = 2 (the dissociation products) is
+// formed as C*C, the value pow gives to within its last bit, at the cost of a
+// product in place of a pow
+HD double conc_pow(double c, float nu)
+{
+   if (nu == 1.0f) return c;
+   if (nu == 2.0f) return c*c;
+   return pow(c, (double)nu);
+}
+
 // ---------------------------------------------------------------- species --
 HD const double *cp_range(const DSpec &s, double T)
 {
@@ -400,8 +411,7 @@ struct Mix {
          double a = 1.0;
          for (int i = 0; i < r.Ne; i++) {
             const int ind = r.ed_ind[i];
-            if (r.ed_nu[i] == 1) a *= C[ind];
-            else                 a *= pow(C[ind], (double)r.ed_nu[i]);
+            a *= conc_pow(C[ind], r.ed_nu[i]);
          }
          double Kb = 0.0, b = 1.0;
          if (r.has_backward) {
@@ -412,8 +422,7 @@ struct Mix {
             Kb = Kf*exp(-lnKc);
             for (int i = 0; i < r.Np; i++) {
                const int ind = r.pd_ind[i];
-               if (r.pd_nu[i] == 1) b *= C[ind];
-               else                 b *= pow(C[ind], (double)r.pd_nu[i]);
+               b *= conc_pow(C[ind], r.pd_nu[i]);
             }
          }
          double R = Kf*a - Kb*b;

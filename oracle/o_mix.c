@@ -400,6 +400,16 @@ static double rate_coeff(const o_arrhenius *a, double T)
    return a->A*exp(arg);
 }
 
+/* C^nu of a rate of progress: Reaction.inl:112-137 multiplies C for nu = 1
+   and takes pow(C, nu) otherwise; nu = 2 is formed as C*C, the value pow
+   gives to within its last bit, at the cost of a product */
+static double conc_pow(double c, float nu)
+{
+   if (nu == 1) return c;
+   if (nu == 2) return c*c;
+   return pow(c, (double)nu);
+}
+
 /* Reaction.inl:82-99 */
 static double backward_coeff(const o_reac *r, double Kf, double P, double T, const double *G)
 {
@@ -454,16 +464,14 @@ static double reaction_rate(const o_reac *r, double P, double T, const double *C
    double a = 1.0;
    for (int i = 0; i < r->Ne; i++) {
       const int ind = r->ed[i].ind;
-      if (r->ed[i].nu == 1) a *= C[ind];
-      else                  a *= pow(C[ind], (double)r->ed[i].nu);
+      a *= conc_pow(C[ind], r->ed[i].nu);
    }
    double Kb = 0.0, b = 1.0;
    if (r->has_backward) {
       Kb = backward_coeff(r, Kf, P, T, G);
       for (int i = 0; i < r->Np; i++) {
          const int ind = r->pd[i].ind;
-         if (r->pd[i].nu == 1) b *= C[ind];
-         else                  b *= pow(C[ind], (double)r->pd[i].nu);
+         b *= conc_pow(C[ind], r->pd[i].nu);
       }
    }
    if (r->kind == O_REAC_THIRDB) {
