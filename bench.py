@@ -354,6 +354,7 @@ def serving(nreq, rate, seed):
     periodic vortices (64^2 .. 256^2), 20-60 SSP-RK3 steps each, admitted into
     packs by free capacity and HBM budget; p50/p99 of the per-step latency."""
     from htr_amd import serving as S
+    S.warm_up()                                   # start-up, untimed: first launches load kernel code
     reqs = S.poisson_requests(nreq, rate, seed=seed, steps=(20, 60), vortex_sizes=(64, 128, 256))
     srv = S.Server(budget_bytes=32 << 30, tube_capacity=1 << 16, vortex_columns=1024)
     stats = srv.run(reqs, progress=lambda m: print(m, file=sys.stderr, flush=True))

@@ -346,6 +346,24 @@ class Server:
                 "packs": {"%s/%d" % k: len(v) for k, v in self.packs.items()}}
 
 
+def warm_up(device="cuda"):
+    """Short cases of each class through a throw-away server: three resident
+    at once per class, then three more admitted while those run, and released
+    in turn.  The first launch of each kernel a step uses (the library's, and
+    torch's concatenations and index copies of the ghost and halo refresh,
+    which differ with one and with several resident cases) loads its code:
+    10-60 ms that a server pays once at start-up, not in a request's step."""
+    reqs, rid = [], 0
+    for arrival, steps in ((0.0, 4), (0.02, 3)):
+        for kind, n in (("tube1", 256), ("tube5", 256), ("vortex", 64)):
+            for k in range(3):
+                reqs.append(Request(rid, kind, n + 32*k, steps + k, arrival))
+                rid += 1
+    Server(budget_bytes=1 << 30, tube_capacity=1 << 12, vortex_columns=512, device=device).run(reqs)
+    if device == "cuda":
+        torch.cuda.synchronize()
+
+
 def poisson_requests(n, rate, seed=0, steps=(20, 60), vortex_sizes=(64, 128, 256)):
     """Seeded Poisson stream: shock tubes (N ~ U{256..16384}, pressure ratio
     ~ U[2, 20], 1 or 5 species) and 2-D vortices (N from vortex_sizes)."""
