@@ -79,6 +79,17 @@ struct DMix {
 
 #define HD __host__ __device__ inline
 
+// One thread's column of a shared species-major array: element i at
+// p[i*stride] (stride = threads of the block, so a wave's lanes touch
+// consecutive words).  The reaction loops index species arrays by runtime
+// species ids; held in registers, each such access becomes a select over
+// every entry, in LDS one address add.
+struct SpeciesCol {
+   double *p;
+   int stride;
+   HD double &operator[](int i) const { return p[i*stride]; }
+};
+
 HD double dmax(double a, double b) { return (a < b) ? b : a; }
 HD double dmin(double a, double b) { return (b < a) ? b : a; }
 
@@ -390,14 +401,15 @@ struct Mix {
       if (fabs((double)EovR) > 1e-5) arg -= (double)EovR/T;
       return A*exp(arg);
    }
-   // MultiComponent.inl:371-417 with Reaction.inl:82-228 (no fall-off)
-   HD static void ProductionRates(const DMix &m, double *w, double rhon, double Pn, double Tn, const double *Y)
+   // MultiComponent.inl:371-417 with Reaction.inl:82-228 (no fall-off), on
+   // species arrays w (rates), C (concentrations) and G (Gibbs energies) of
+   // type A: per-thread arrays, or SpeciesCol columns of a shared array
+   template<class A>
+   HD static void ProductionRatesIn(const DMix &m, A w, A C, A G, double rhon, double Pn, double Tn, const double *Y)
    {
-      if (CP) { w[0] = 0.0; return; }
       const double T = Tn*m.TRef;
       const double P = Pn*m.PRef;
       const double rho = rhon*m.rhoRef;
-      double G[NS], C[NS];
       for (int i = 0; i < NS; i++) {
          w[i] = 0.0;
          C[i] = Y[i]*rho/m.sp[i].W;
@@ -438,6 +450,12 @@ struct Mix {
          for (int i = 0; i < r.Np; i++) w[r.pd_ind[i]] += r.pd_nu[i]*R;
       }
       for (int i = 0; i < NS; i++) w[i] *= m.sp[i].W*m.iwiRef;
+   }
+   HD static void ProductionRates(const DMix &m, double *w, double rhon, double Pn, double Tn, const double *Y)
+   {
+      if (CP) { w[0] = 0.0; return; }
+      double G[NS], C[NS];
+      ProductionRatesIn<double *>(m, w, C, G, rhon, Pn, Tn, Y);
    }
 };
 

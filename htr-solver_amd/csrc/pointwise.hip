@@ -233,7 +233,18 @@ chem_source_kernel(const htr_fields f, const DMix *__restrict__ mix, const Geo g
    double Y[NS], w[NS];
 #pragma unroll
    for (int i = 0; i < NS; i++) Y[i] = ld<double>(f.MassFracs, p, i);
-   MX::ProductionRates(*mix, w, ld<double>(f.rho, p), ld<double>(f.pressure, p), ld<double>(f.temperature, p), Y);
+   const double rho = ld<double>(f.rho, p), P = ld<double>(f.pressure, p), T = ld<double>(f.temperature, p);
+   if constexpr (!MX::CP && NS <= 8) {
+      // rates, concentrations and Gibbs energies in this thread's LDS columns
+      // (3 NS 2 KiB per block)
+      __shared__ double sp[3*NS*256];
+      const SpeciesCol W{sp + threadIdx.x, 256}, C{sp + NS*256 + threadIdx.x, 256}, G{sp + 2*NS*256 + threadIdx.x, 256};
+      MX::ProductionRatesIn(*mix, W, C, G, rho, P, T, Y);
+#pragma unroll
+      for (int i = 0; i < NS; i++) w[i] = W[i];
+   } else {
+      MX::ProductionRates(*mix, w, rho, P, T, Y);
+   }
    if (zero) {
       // first term of the stage's RHS: Conserved_t = 0 + source, the other
       // components cleared (the zeroing of InitializeTimeDerivatives fused)
