@@ -351,11 +351,11 @@ def chemistry(n, dt, seed, cpu, threads, tune=True):
 def serving(nreq, rate, seed):
     """Continuous-batching request loop: a seeded Poisson stream of shock tubes
     (N ~ U{256..16384}, pressure ratio U[2, 20], 1 or 5 species) and 2-D
-    periodic vortices (64^2 .. 256^2), 20-60 SSP-RK3 steps each, admitted into
+    periodic vortices (64^2 .. 1024^2), 20-60 SSP-RK3 steps each, admitted into
     packs by free capacity and HBM budget; p50/p99 of the per-step latency."""
     from htr_amd import serving as S
     S.warm_up()                                   # start-up, untimed: first launches load kernel code
-    reqs = S.poisson_requests(nreq, rate, seed=seed, steps=(20, 60), vortex_sizes=(64, 128, 256))
+    reqs = S.poisson_requests(nreq, rate, seed=seed, steps=(20, 60), vortex_sizes=(64, 128, 256, 512, 1024))
     srv = S.Server(budget_bytes=32 << 30, tube_capacity=1 << 16, vortex_columns=1024)
     stats = srv.run(reqs, progress=lambda m: print(m, file=sys.stderr, flush=True))
     stats.update({"arrival_rate_per_s": rate, "seed": seed,
